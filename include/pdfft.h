@@ -1,0 +1,150 @@
+/*
+ * pdfft.h -- C ABI of libpdfft.so, the MI355X (gfx950) FFT rigid-body
+ * correlation scan.
+ *
+ * The reference (andreyto/proddl2) has no C ABI: its scan is the header-only
+ * template Docking<T>::Worker compiled into the proddl-dock-fft executable.
+ * Each entry point below replaces one in-process seam of that executable; the
+ * seam it replaces is cited next to it.  Plain pointers and sizes only; no
+ * exceptions cross this boundary.  Every call returns a pd_status and, on
+ * failure, pd_last_error() holds a message (thread-local).
+ *
+ * Ownership: the caller owns every buffer passed in or out.  A handle owns its
+ * device memory and one HIP stream (default priority).  A handle is not
+ * thread-safe; separate handles are independent.
+ *
+ * Conventions (all from the reference):
+ *  - grids are N0 x N1 x N2, C order, last index fastest;
+ *  - the geometry is centred: cell of x is int((x - zero_h)/h) - 1 with
+ *    zero_h = -(h + h*(N-1))/2 - h  (Grid/gridcoord.hpp:127-142, 185-192);
+ *  - energies: lower is better; a translation is kept only if value < maxValCorr;
+ *  - displacement s of a correlation offset x: s = -x, wrapped into
+ *    [-floor(N/2), N - floor(N/2) - 1] (Math/fft.hpp:199-206); translation = h*s.
+ */
+#ifndef PDFFT_H
+#define PDFFT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum pd_status {
+  PD_OK = 0,
+  PD_ERR_INVALID_ARGUMENT = 1, /* dbg_exception from ATALWAYS (Common/debug.hpp:55,68-71) */
+  PD_ERR_DOMAIN = 2,           /* grid_domain_error (Grid/grid.hpp:526): atom outside grid */
+  PD_ERR_IO = 3,               /* file errors (docking_io_bin.hpp:27)                    */
+  PD_ERR_DEVICE = 4,           /* HIP runtime failure (no reference counterpart)          */
+  PD_ERR_UNSUPPORTED = 5,      /* not_supported_error (docking_app.hpp:65)                */
+  PD_ERR_STATE = 6             /* call order violated (e.g. run before receptor set)      */
+} pd_status;
+
+/* Radial field kinds for receptor terms (MolForceRadialR plugins,
+ * docking_molforce.hpp:287-374).  Per-atom parameter counts in brackets. */
+enum {
+  PD_FIELD_LJ_REP = 0,  /* SoftCoreLJRep::f2 (potentials_inc.hpp:311-314)  [e4s12, as6]   */
+  PD_FIELD_LJ_ATT = 1,  /* SoftCoreLJAttr::f2 (potentials_inc.hpp:349-352) [e4s6, as6]    */
+  PD_FIELD_LJ_FULL = 2, /* SoftCoreLJ::f2 (potentials_inc.hpp:214-219) [e4s12,e4s6,as6,fshift,gshift] */
+  PD_FIELD_TWOSTEP = 3, /* new shape term: v_in if r2<=r_in2 else v_out   [v_in, v_out, r_in2] */
+  PD_FIELD_COULOMB = 4, /* new: kq/r2, distance-dependent dielectric     [kq]                */
+  PD_FIELD_DESOLV = 5   /* new: d*(1-r2/cut2)^2 short-range desolvation   [d]                 */
+};
+
+typedef struct pd_scan_desc {
+  int n[3];               /* FFT grid edges (FFTCorrelator::init, docking_fft.hpp:545-553) */
+  double grid_step;       /* option gridStep (docking_app.hpp:72)                           */
+  int n_terms;            /* MolForce::nGrids() (docking_molforce.hpp:83)                   */
+  int max_n_trans;        /* option maxNTrans: peaks kept per rotation (docking_app.hpp:84), 1..64 */
+  int max_n_trans_inp;    /* option maxNTransInp (docking_app.hpp:374); >= max_n_trans       */
+  double max_val_corr;    /* option maxValCorr (docking_app.hpp:363)                        */
+  int batch;              /* rotations per kernel launch (0: 64)                             */
+  int device;             /* HIP device ordinal                                              */
+} pd_scan_desc;
+
+typedef struct pd_term {
+  int kind;               /* PD_FIELD_*                                                      */
+  double cutoff;          /* field radius (option cutOffFft for ljComp)                      */
+  double min_dist;        /* r is clamped to >= min_dist; < 0 means grid_step/4
+                             (docking_molforce.hpp:307-310)                                  */
+  const double* params;   /* n_atoms x nparams(kind), row-major                              */
+} pd_term;
+
+/* One selected translation of one rotation (TranValue, types.hpp:70-76, made explicit). */
+typedef struct pd_peak {
+  int32_t rot;            /* global rotation index = rot_index0 + position in the call     */
+  int32_t rank;           /* 0-based rank within the rotation (ascending value)            */
+  int32_t s[3];           /* displacement in cells                                          */
+  int32_t offset;         /* row-major offset of the correlation cell                       */
+  float value;            /* FP32 correlation energy                                        */
+  float reserved;
+  double value64;         /* FP64 rescored energy, NaN until pd_scan_rescore64 fills it     */
+} pd_peak;
+
+/* FFTCorrelators ctor + CorrelationProcessor::init (docking_fft.hpp:771-795, 283-312). */
+pd_status pd_scan_create(const pd_scan_desc* desc, struct pd_scan** out);
+
+/* MolForce::projectMol(iRec) + FFTCorrelators::preprocessReceptor
+ * (docking_app.hpp:406-415; Grid/project.hpp:143-176; docking_fft.hpp:614-618):
+ * FP64 projection of n_terms radial fields on the GPU, then FP32 r2c spectra. */
+pd_status pd_scan_project_receptor(struct pd_scan* h, int n_atoms, const double* pos,
+                                   const pd_term* terms);
+
+/* Same as above from caller-projected FP64 grids (n_terms x N0*N1*N2). */
+pd_status pd_scan_set_receptor_grids(struct pd_scan* h, const double* grids);
+
+/* Copies the FP64 receptor grids back (n_terms x N0*N1*N2). */
+pd_status pd_scan_get_receptor_grids(struct pd_scan* h, double* grids);
+
+/* MolForceRadial ligand data (docking_molforce.hpp:351-372): reference-frame
+ * coordinates (n x 3, FP64) and per-term weights (n_terms x n, FP64). */
+pd_status pd_scan_set_ligand(struct pd_scan* h, int n_atoms, const double* pos,
+                             const double* weights);
+
+/* Worker::run over a rotation slice (docking_app.hpp:487-513 via 450-466, 417-447):
+ * rots is n_rot x 9 row-major FP64 (host); peaks is n_rot x max_n_trans (host);
+ * n_out[i] = peaks written for rotation i. */
+pd_status pd_scan_run(struct pd_scan* h, int n_rot, const double* rots, int rot_index0,
+                      pd_peak* peaks, int* n_out);
+
+/* Same with device-resident rotations and outputs; asynchronous on the handle's
+ * stream unless sync != 0.  Domain errors are reported by pd_scan_sync. */
+pd_status pd_scan_run_device(struct pd_scan* h, int n_rot, const double* d_rots,
+                             int rot_index0, pd_peak* d_peaks, int* d_n_out, int sync);
+pd_status pd_scan_sync(struct pd_scan* h);
+
+/* FP64 rescoring (new; no reference counterpart): E = sum_t sum_j w_tj R_t(l_j + s)
+ * from the FP64 receptor grids, for n poses (rots n x 9, s n x 3, host). */
+pd_status pd_scan_rescore64(struct pd_scan* h, int n, const double* rots, const int* s,
+                            double* energies);
+
+/* The summed correlation grid of one rotation (FP32, N0*N1*N2), for parity tests
+ * (the gridTot after collectTotal, docking_app.hpp:427). */
+pd_status pd_scan_correlation_grid(struct pd_scan* h, const double* rot, float* out);
+
+/* Per-kernel device time recorded with HIP events on the handle's stream.
+ * enable != 0 starts recording; stats returns, for kernel id k
+ * (0 bin, 1 ligand plane forward, 2 column, 3 inverse plane + top-K, 4 merge),
+ * total milliseconds and launch count. */
+pd_status pd_scan_set_timing(struct pd_scan* h, int enable);
+pd_status pd_scan_kernel_stats(struct pd_scan* h, int k, double* total_ms, long long* launches);
+void* pd_scan_stream(struct pd_scan* h);
+
+void pd_scan_destroy(struct pd_scan* h);
+const char* pd_last_error(void);
+
+/* FFTW_Size::findBestSize (Math/fft.hpp:141-151); returns -1 if none. */
+int pd_find_best_size(int start);
+
+/* Per-atom ljComp parameters: MolForceLJRep / MolForceLJAttr ctors
+ * (docking_molforce.hpp:461-581).  rep/att: n_rec x 2; w_rep, w_att: n_lig. */
+pd_status pd_ljcomp_params(int n_rec, const double* eps_r, const double* sigma_r,
+                           const double* alpha_r, int n_lig, const double* eps_l,
+                           const double* sigma_l, double* rep, double* att, double* w_rep,
+                           double* w_att);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PDFFT_H */

@@ -1,0 +1,674 @@
+// pdfft.hip -- host side of libpdfft.so: the C ABI declared in include/pdfft.h.
+//
+// A handle (pd_scan) stands for one FFTCorrelators + CorrelationProcessor pair
+// of the reference worker (docking_fft.hpp:236-492, 767-883): the receptor
+// spectra of every term stay resident in HBM, rotations stream through in
+// batches, and each batch runs five kernels (scan_kernels.hpp).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <vector>
+
+#include "../../include/pdfft.h"
+#include "scan_kernels.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+
+pd_status fail(pd_status s, const std::string& msg) {
+  g_err = msg;
+  return s;
+}
+
+#define PD_HIP(x)                                                                       \
+  do {                                                                                  \
+    hipError_t e_ = (x);                                                                \
+    if (e_ != hipSuccess)                                                               \
+      return fail(PD_ERR_DEVICE, std::string(#x) + " failed: " + hipGetErrorString(e_)); \
+  } while (0)
+
+constexpr int kPlaneThreads = 512;
+constexpr int kColThreads = 256;
+constexpr int kMaxLds = 160 * 1024;
+constexpr int kMaxK = 32;
+
+int field_nparams(int kind) {
+  switch (kind) {
+    case PD_FIELD_LJ_REP: return 2;
+    case PD_FIELD_LJ_ATT: return 2;
+    case PD_FIELD_LJ_FULL: return 5;
+    case PD_FIELD_TWOSTEP: return 3;
+    case PD_FIELD_COULOMB: return 1;
+    case PD_FIELD_DESOLV: return 1;
+  }
+  return -1;
+}
+
+// Centred geometry of one axis (Grid/gridcoord.hpp:127-142 then setup() 166-175).
+void centred_axis(double h, int n, double* zero_h, double* zsc) {
+  double upper = 0.0 + h + h * (n - 1);
+  double lower = 0.0 + h * 0;
+  double zero = -1. * (upper - lower) / 2;
+  *zero_h = zero - h;
+  *zsc = (zero - h * 0) + h / 2.0;
+}
+
+int to_logical_h(double x, double zero_h, double h) {
+  double u = (x - zero_h) / h;
+  if (!(u > -1.0e9 && u < 1.0e9)) return std::numeric_limits<int>::min();
+  return static_cast<int>(u) - 1;
+}
+
+// Radix sequence for an FFT length: 16, 8, 4, 2 first, then odd primes <= 13.
+bool radices_for(int n, std::vector<int>& r) {
+  r.clear();
+  while (n % 16 == 0) { r.push_back(16); n /= 16; }
+  if (n % 8 == 0) { r.push_back(8); n /= 8; }
+  if (n % 4 == 0) { r.push_back(4); n /= 4; }
+  if (n % 2 == 0) { r.push_back(2); n /= 2; }
+  for (int p : {3, 5, 7, 11, 13})
+    while (n % p == 0) { r.push_back(p); n /= p; }
+  return n == 1 && (int)r.size() <= PD_MAX_STAGES;
+}
+
+// Host copy of a plan's stages; the device copy is uploaded by upload_plan.
+struct HostPlan {
+  int n = 0;
+  std::vector<int4> st;
+};
+
+bool make_plan(int n, HostPlan& P) {
+  std::vector<int> r;
+  if (n < 1 || !radices_for(n, r)) return false;
+  P.n = n;
+  P.st.clear();
+  int ns = 1;
+  for (size_t i = 0; i < r.size(); ++i) {
+    P.st.push_back(make_int4(r[i], ns, n / (ns * r[i]), 0));
+    ns *= r[i];
+  }
+  return true;
+}
+
+// Each thread keeps at most PD_MAXV/R butterflies of a radix-R stage in registers.
+bool plan_fits(const HostPlan& P, int lines, int threads) {
+  for (size_t s = 0; s < P.st.size(); ++s) {
+    const int R = P.st[s].x;
+    const int maxt = std::max(1, PD_MAXV / R);
+    const long long tasks = (long long)lines * (P.n / R);
+    if ((tasks + threads - 1) / threads > maxt) return false;
+  }
+  return true;
+}
+
+}  // namespace
+
+struct pd_scan {
+  pd_scan_desc d{};
+  hipStream_t stream = nullptr;
+  ScanGeom g{};
+  int K = 10, B = 64, C = 32;
+  long long N = 0;
+  HostPlan hx, hy, hz;
+  FftPlan px{}, py{}, pz{};
+  int4* d_stages = nullptr;  // stage tables of px, py, pz
+  float2 *tw_x = nullptr, *tw_y = nullptr, *tw_z = nullptr, *tw2 = nullptr;
+  double* R64 = nullptr;
+  float2* Rhat = nullptr;
+  bool receptor_ready = false;
+  int n_lig = 0;
+  double* q = nullptr;
+  double* w64 = nullptr;
+  float* w32 = nullptr;
+  bool ligand_ready = false;
+  int* cells = nullptr;
+  float2* L1 = nullptr;
+  float2* S1 = nullptr;
+  float* cval = nullptr;
+  int* coff = nullptr;
+  int* err = nullptr;
+  double* d_rots = nullptr;
+  size_t rots_cap = 0;
+  PeakOut* d_peaks = nullptr;
+  int* d_nout = nullptr;
+  size_t out_cap = 0;
+  bool timing = false;
+  struct Pending { int k; hipEvent_t a, b; };
+  std::vector<Pending> pending;
+  double tot_ms[5] = {0, 0, 0, 0, 0};
+  long long launches[5] = {0, 0, 0, 0, 0};
+};
+
+namespace {
+
+void free_batch(pd_scan* h) {
+  hipFree(h->cells); h->cells = nullptr;
+  hipFree(h->L1); h->L1 = nullptr;
+  hipFree(h->S1); h->S1 = nullptr;
+  hipFree(h->cval); h->cval = nullptr;
+  hipFree(h->coff); h->coff = nullptr;
+}
+
+pd_status make_twiddles(pd_scan* h, int n, float2** out) {
+  PD_HIP(hipMalloc(out, sizeof(float2) * (size_t)n));
+  hipLaunchKernelGGL(k_twiddles, dim3((n + 255) / 256), dim3(256), 0, h->stream, n, *out);
+  PD_HIP(hipGetLastError());
+  return PD_OK;
+}
+
+pd_status set_lds(const void* fn, int bytes) {
+  if (bytes > kMaxLds) return fail(PD_ERR_UNSUPPORTED, "LDS requirement exceeds 160 KiB");
+  PD_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+  return PD_OK;
+}
+
+struct Timer {
+  pd_scan* h;
+  int k;
+  hipEvent_t a = nullptr, b = nullptr;
+  Timer(pd_scan* h_, int k_) : h(h_), k(k_) {
+    if (h->timing) {
+      hipEventCreate(&a);
+      hipEventCreate(&b);
+      hipEventRecord(a, h->stream);
+    }
+  }
+  ~Timer() {
+    if (h->timing) {
+      hipEventRecord(b, h->stream);
+      h->pending.push_back({k, a, b});
+    }
+  }
+};
+
+pd_status resolve_timing(pd_scan* h) {
+  if (h->pending.empty()) return PD_OK;
+  PD_HIP(hipStreamSynchronize(h->stream));
+  for (auto& p : h->pending) {
+    float ms = 0;
+    PD_HIP(hipEventElapsedTime(&ms, p.a, p.b));
+    h->tot_ms[p.k] += ms;
+    h->launches[p.k] += 1;
+    hipEventDestroy(p.a);
+    hipEventDestroy(p.b);
+  }
+  h->pending.clear();
+  return PD_OK;
+}
+
+int plane_lds(const pd_scan* h) { return h->g.n1 * h->g.ls * (int)sizeof(float2); }
+int col_lds(const pd_scan* h) { return h->g.n0 * h->C * (int)sizeof(float2); }
+
+// Receptor forward transform of the FP64 grids already in h->R64.
+pd_status receptor_spectra(pd_scan* h) {
+  const ScanGeom& g = h->g;
+  float* r32 = nullptr;
+  float2* tmp = nullptr;
+  PD_HIP(hipMalloc(&r32, sizeof(float) * h->N * g.T));
+  PD_HIP(hipMalloc(&tmp, sizeof(float2) * (size_t)g.n0 * g.M));
+  const long long tot = h->N * g.T;
+  hipLaunchKernelGGL(k_to_float, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, h->stream, h->R64,
+                     r32, tot);
+  PD_HIP(hipGetLastError());
+  const int ntile = (g.M + h->C - 1) / h->C;
+  for (int t = 0; t < g.T; ++t) {
+    hipLaunchKernelGGL(k_rec_plane_fwd, dim3(g.n0), dim3(kPlaneThreads), plane_lds(h), h->stream, g,
+                       h->pz, h->py, h->tw2, r32 + (size_t)t * h->N, tmp);
+    PD_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_col_fwd, dim3(ntile), dim3(kColThreads), col_lds(h), h->stream, g, h->px,
+                       tmp, h->Rhat + (size_t)t * g.n0 * g.M, h->C);
+    PD_HIP(hipGetLastError());
+  }
+  PD_HIP(hipStreamSynchronize(h->stream));
+  hipFree(r32);
+  hipFree(tmp);
+  h->receptor_ready = true;
+  return PD_OK;
+}
+
+pd_status run_batches(pd_scan* h, int n_rot, const double* d_rots, int rot0, PeakOut* d_peaks,
+                      int* d_nout) {
+  if (!h->receptor_ready) return fail(PD_ERR_STATE, "receptor not set");
+  if (!h->ligand_ready) return fail(PD_ERR_STATE, "ligand not set");
+  if (n_rot < 0) return fail(PD_ERR_INVALID_ARGUMENT, "n_rot < 0");
+  const ScanGeom& g = h->g;
+  const int ntile = (g.M + h->C - 1) / h->C;
+  const float scale = (float)(1.0 / (double)h->N);
+  for (int b0 = 0; b0 < n_rot; b0 += h->B) {
+    const int nb = std::min(h->B, n_rot - b0);
+    {
+      Timer tm(h, 0);
+      const long long tot = (long long)nb * h->n_lig;
+      hipLaunchKernelGGL(k_bin, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, h->stream, g,
+                         d_rots + 9 * (size_t)b0, nb, h->q, h->n_lig, h->cells, h->err);
+    }
+    {
+      Timer tm(h, 1);
+      hipLaunchKernelGGL(k_lig_plane_fwd, dim3(g.p[0], nb), dim3(kPlaneThreads), plane_lds(h),
+                         h->stream, g, h->pz, h->py, h->tw2, h->cells, h->n_lig, h->w32, h->L1);
+    }
+    {
+      Timer tm(h, 2);
+      int nyb = std::max(1, std::min(nb, 4096 / std::max(1, ntile)));
+      int bpb = (nb + nyb - 1) / nyb;
+      nyb = (nb + bpb - 1) / bpb;
+      hipLaunchKernelGGL(k_col, dim3(ntile, nyb), dim3(kColThreads), col_lds(h), h->stream, g, h->px,
+                         h->L1, h->Rhat, h->S1, h->C, bpb, nb);
+    }
+    {
+      Timer tm(h, 3);
+      hipLaunchKernelGGL(k_plane_inv_topk, dim3(g.n0, nb), dim3(kPlaneThreads), plane_lds(h),
+                         h->stream, g, h->pz, h->py, h->tw2, h->S1, scale, h->K, h->cval, h->coff,
+                         (float*)nullptr);
+    }
+    {
+      Timer tm(h, 4);
+      hipLaunchKernelGGL(k_merge, dim3(nb), dim3(256), (size_t)g.n0 * h->K * 8, h->stream, g, h->K,
+                         h->cval, h->coff, h->d.max_val_corr, rot0 + b0, d_peaks + (size_t)b0 * h->K,
+                         d_nout + b0);
+    }
+    PD_HIP(hipGetLastError());
+  }
+  return PD_OK;
+}
+
+pd_status check_err(pd_scan* h) {
+  int e = 0;
+  PD_HIP(hipMemcpyAsync(&e, h->err, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  PD_HIP(hipStreamSynchronize(h->stream));
+  if (e) {
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(1), 0, h->stream, h->err, 1, 0);
+    return fail(PD_ERR_DOMAIN, "a rotated ligand atom fell outside the FFT grid");
+  }
+  return PD_OK;
+}
+
+pd_status ensure_io(pd_scan* h, int n_rot) {
+  if ((size_t)n_rot > h->rots_cap) {
+    hipFree(h->d_rots);
+    PD_HIP(hipMalloc(&h->d_rots, sizeof(double) * 9 * (size_t)n_rot));
+    h->rots_cap = n_rot;
+  }
+  if ((size_t)n_rot > h->out_cap) {
+    hipFree(h->d_peaks);
+    hipFree(h->d_nout);
+    PD_HIP(hipMalloc(&h->d_peaks, sizeof(PeakOut) * (size_t)n_rot * h->K));
+    PD_HIP(hipMalloc(&h->d_nout, sizeof(int) * (size_t)n_rot));
+    h->out_cap = n_rot;
+  }
+  return PD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* pd_last_error(void) { return g_err.c_str(); }
+
+int pd_find_best_size(int start) {
+  // FFTW_Size::findBestSize (Math/fft.hpp:82-151): first n >= max(2, start)
+  // factoring over {2,3,5,7,11,13} with e11 + e13 <= 1.
+  for (long long n = std::max(2, start); n < std::numeric_limits<int>::max(); ++n) {
+    long long m = n;
+    int e[6] = {0, 0, 0, 0, 0, 0};
+    const int pr[6] = {2, 3, 5, 7, 11, 13};
+    for (int i = 0; i < 6; ++i)
+      while (m % pr[i] == 0) { m /= pr[i]; ++e[i]; }
+    if (m == 1 && e[4] + e[5] <= 1) return (int)n;
+  }
+  return -1;
+}
+
+pd_status pd_ljcomp_params(int n_rec, const double* eps_r, const double* sigma_r,
+                           const double* alpha_r, int n_lig, const double* eps_l,
+                           const double* sigma_l, double* rep, double* att, double* w_rep,
+                           double* w_att) {
+  if (n_rec < 0 || n_lig <= 0) return fail(PD_ERR_INVALID_ARGUMENT, "atom counts");
+  double sl = 0;
+  for (int j = 0; j < n_lig; ++j) sl += sigma_l[j];
+  const double sig_avg = sl / n_lig;
+  for (int a = 0; a < n_rec; ++a) {
+    const double sg = std::sqrt(sigma_r[a]), sa = std::sqrt(sigma_r[a] * sig_avg),
+                 ep = std::sqrt(eps_r[a]);
+    const double t1 = sg * sg * sg, s6 = t1 * t1;
+    const double t2 = sa * sa * sa, sa6 = t2 * t2;
+    const double as6 = alpha_r[a] * sa6;
+    const double e4s6 = 4. * ep * s6;
+    rep[2 * a] = e4s6 * s6;
+    rep[2 * a + 1] = as6;
+    att[2 * a] = e4s6;
+    att[2 * a + 1] = as6;
+  }
+  for (int j = 0; j < n_lig; ++j) {
+    const double s = sigma_l[j], e = std::sqrt(eps_l[j]);
+    const double t3 = s * s * s;
+    w_rep[j] = e * (t3 * t3);
+    w_att[j] = e * t3;
+  }
+  return PD_OK;
+}
+
+pd_status pd_scan_create(const pd_scan_desc* desc, pd_scan** out) {
+  if (!desc || !out) return fail(PD_ERR_INVALID_ARGUMENT, "null argument");
+  *out = nullptr;
+  const pd_scan_desc& d = *desc;
+  for (int i = 0; i < 3; ++i)
+    if (d.n[i] < 2 || d.n[i] > 4096) return fail(PD_ERR_INVALID_ARGUMENT, "grid edge out of range");
+  if (d.n[2] % 2 != 0) return fail(PD_ERR_UNSUPPORTED, "odd last grid edge is not supported");
+  if (!(d.grid_step > 0)) return fail(PD_ERR_INVALID_ARGUMENT, "grid_step must be > 0");
+  if (d.n_terms < 1 || d.n_terms > 16) return fail(PD_ERR_INVALID_ARGUMENT, "n_terms must be 1..16");
+  if (d.max_n_trans < 1 || d.max_n_trans > kMaxK)
+    return fail(PD_ERR_UNSUPPORTED, "max_n_trans must be 1..32");
+  if (d.max_n_trans_inp < d.max_n_trans)
+    return fail(PD_ERR_INVALID_ARGUMENT, "max_n_trans_inp < max_n_trans");
+  if (d.batch < 0) return fail(PD_ERR_INVALID_ARGUMENT, "batch < 0");
+  int ndev = 0;
+  PD_HIP(hipGetDeviceCount(&ndev));
+  if (d.device < 0 || d.device >= ndev) return fail(PD_ERR_DEVICE, "no such HIP device");
+  PD_HIP(hipSetDevice(d.device));
+
+  pd_scan* h = new pd_scan();
+  h->d = d;
+  h->K = d.max_n_trans;
+  h->B = d.batch > 0 ? d.batch : 64;
+  ScanGeom& g = h->g;
+  g.n0 = d.n[0];
+  g.n1 = d.n[1];
+  g.n2 = d.n[2];
+  g.h2 = g.n2 / 2 + 1;
+  g.m2 = g.n2 / 2;
+  g.ls = g.h2 | 1;
+  g.M = g.n1 * g.h2;
+  g.T = d.n_terms;
+  g.h = d.grid_step;
+  for (int a = 0; a < 3; ++a) {
+    centred_axis(d.grid_step, d.n[a], &g.zero_h[a], &g.zsc[a]);
+    g.lo[a] = 0;
+    g.p[a] = d.n[a];
+  }
+  h->N = (long long)g.n0 * g.n1 * g.n2;
+  h->C = std::max(1, std::min(64, 4096 / g.n0));
+  auto bad = [&](pd_status s, const std::string& m) {
+    pd_scan_destroy(h);
+    return fail(s, m);
+  };
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
+    return bad(PD_ERR_DEVICE, "hipStreamCreate failed");
+  pd_status st;
+  if ((st = make_twiddles(h, g.n0, &h->tw_x)) != PD_OK) return bad(st, g_err);
+  if ((st = make_twiddles(h, g.n1, &h->tw_y)) != PD_OK) return bad(st, g_err);
+  if ((st = make_twiddles(h, g.m2, &h->tw_z)) != PD_OK) return bad(st, g_err);
+  if ((st = make_twiddles(h, g.n2, &h->tw2)) != PD_OK) return bad(st, g_err);
+  if (!make_plan(g.n0, h->hx) || !make_plan(g.n1, h->hy) || !make_plan(g.m2, h->hz))
+    return bad(PD_ERR_UNSUPPORTED, "grid edges must factor over {2,3,5,7,11,13}");
+  if (!plan_fits(h->hy, g.h2, kPlaneThreads) || !plan_fits(h->hz, g.n1, kPlaneThreads) ||
+      !plan_fits(h->hx, h->C, kColThreads) || g.n0 * h->C > PD_COL_PER * kColThreads)
+    return bad(PD_ERR_UNSUPPORTED, "grid plane too large for the register plan");
+  {
+    std::vector<int4> all;
+    all.insert(all.end(), h->hx.st.begin(), h->hx.st.end());
+    all.insert(all.end(), h->hy.st.begin(), h->hy.st.end());
+    all.insert(all.end(), h->hz.st.begin(), h->hz.st.end());
+    if (hipMalloc(&h->d_stages, sizeof(int4) * all.size()) != hipSuccess ||
+        hipMemcpy(h->d_stages, all.data(), sizeof(int4) * all.size(), hipMemcpyHostToDevice) != hipSuccess)
+      return bad(PD_ERR_DEVICE, "stage table upload failed");
+    const int nx = (int)h->hx.st.size(), ny = (int)h->hy.st.size(), nz = (int)h->hz.st.size();
+    h->px = FftPlan{g.n0, nx, h->d_stages, h->tw_x};
+    h->py = FftPlan{g.n1, ny, h->d_stages + nx, h->tw_y};
+    h->pz = FftPlan{g.m2, nz, h->d_stages + nx + ny, h->tw_z};
+  }
+  if ((st = set_lds((const void*)k_lig_plane_fwd, plane_lds(h))) != PD_OK) return bad(st, g_err);
+  if ((st = set_lds((const void*)k_plane_inv_topk, plane_lds(h))) != PD_OK) return bad(st, g_err);
+  if ((st = set_lds((const void*)k_rec_plane_fwd, plane_lds(h))) != PD_OK) return bad(st, g_err);
+  if ((st = set_lds((const void*)k_col, col_lds(h))) != PD_OK) return bad(st, g_err);
+  if ((st = set_lds((const void*)k_col_fwd, col_lds(h))) != PD_OK) return bad(st, g_err);
+  if ((st = set_lds((const void*)k_merge, g.n0 * h->K * 8)) != PD_OK) return bad(st, g_err);
+  if (hipMalloc(&h->R64, sizeof(double) * h->N * g.T) != hipSuccess ||
+      hipMalloc(&h->Rhat, sizeof(float2) * (size_t)g.T * g.n0 * g.M) != hipSuccess ||
+      hipMalloc(&h->err, sizeof(int)) != hipSuccess)
+    return bad(PD_ERR_DEVICE, "device allocation failed");
+  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(1), 0, h->stream, h->err, 1, 0);
+  if (hipStreamSynchronize(h->stream) != hipSuccess) return bad(PD_ERR_DEVICE, "stream sync failed");
+  *out = h;
+  return PD_OK;
+}
+
+void pd_scan_destroy(pd_scan* h) {
+  if (!h) return;
+  if (h->stream) hipStreamSynchronize(h->stream);
+  for (auto& p : h->pending) { hipEventDestroy(p.a); hipEventDestroy(p.b); }
+  free_batch(h);
+  hipFree(h->tw_x); hipFree(h->tw_y); hipFree(h->tw_z); hipFree(h->tw2); hipFree(h->d_stages);
+  hipFree(h->R64); hipFree(h->Rhat);
+  hipFree(h->q); hipFree(h->w64); hipFree(h->w32);
+  hipFree(h->err); hipFree(h->d_rots); hipFree(h->d_peaks); hipFree(h->d_nout);
+  if (h->stream) hipStreamDestroy(h->stream);
+  delete h;
+}
+
+pd_status pd_scan_project_receptor(pd_scan* h, int n_atoms, const double* pos, const pd_term* terms) {
+  if (!h || !pos || !terms || n_atoms < 1) return fail(PD_ERR_INVALID_ARGUMENT, "bad receptor arguments");
+  PD_HIP(hipSetDevice(h->d.device));
+  const ScanGeom& g = h->g;
+  double* dpos = nullptr;
+  int* dreg = nullptr;
+  double* dpar = nullptr;
+  PD_HIP(hipMalloc(&dpos, sizeof(double) * 3 * n_atoms));
+  PD_HIP(hipMalloc(&dreg, sizeof(int) * 6 * n_atoms));
+  PD_HIP(hipMalloc(&dpar, sizeof(double) * 5 * n_atoms));
+  PD_HIP(hipMemcpy(dpos, pos, sizeof(double) * 3 * n_atoms, hipMemcpyHostToDevice));
+  std::vector<int> reg(6 * (size_t)n_atoms);
+  const int nb = ((g.n0 + 3) / 4) * ((g.n1 + 7) / 8) * ((g.n2 + 7) / 8);
+  pd_status st = PD_OK;
+  for (int t = 0; t < g.T && st == PD_OK; ++t) {
+    const pd_term& tm = terms[t];
+    const int np = field_nparams(tm.kind);
+    if (np < 0 || !tm.params || !(tm.cutoff > 0)) { st = fail(PD_ERR_INVALID_ARGUMENT, "bad term"); break; }
+    const double cut = tm.cutoff;
+    const double md = tm.min_dist < 0 ? g.h / 4. : tm.min_dist;
+    // GridRegionProjectionHelper (Grid/project.hpp:89-98) + cropSubDomain (grid.hpp:613-620)
+    for (int a = 0; a < n_atoms; ++a) {
+      for (int dd = 0; dd < 3; ++dd) {
+        const double c = pos[3 * a + dd];
+        int lo = to_logical_h(c - cut, g.zero_h[dd], g.h);
+        int hi = to_logical_h(c + cut, g.zero_h[dd], g.h);
+        lo = std::max(0, lo);
+        hi = std::min(h->d.n[dd] - 1, hi);
+        reg[6 * a + 2 * dd] = lo;
+        reg[6 * a + 2 * dd + 1] = hi;
+      }
+    }
+    if (hipMemcpy(dreg, reg.data(), sizeof(int) * reg.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dpar, tm.params, sizeof(double) * np * (size_t)n_atoms, hipMemcpyHostToDevice) !=
+            hipSuccess) {
+      st = fail(PD_ERR_DEVICE, "receptor upload failed");
+      break;
+    }
+    hipLaunchKernelGGL(k_project, dim3(nb), dim3(256), 0, h->stream, g, n_atoms, dpos, dreg, tm.kind,
+                       dpar, np, cut * cut, md * md, h->R64 + (size_t)t * h->N);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(h->stream) != hipSuccess)
+      st = fail(PD_ERR_DEVICE, "receptor projection failed");
+  }
+  hipFree(dpos);
+  hipFree(dreg);
+  hipFree(dpar);
+  if (st != PD_OK) return st;
+  return receptor_spectra(h);
+}
+
+pd_status pd_scan_set_receptor_grids(pd_scan* h, const double* grids) {
+  if (!h || !grids) return fail(PD_ERR_INVALID_ARGUMENT, "null argument");
+  PD_HIP(hipSetDevice(h->d.device));
+  PD_HIP(hipMemcpy(h->R64, grids, sizeof(double) * h->N * h->g.T, hipMemcpyHostToDevice));
+  return receptor_spectra(h);
+}
+
+pd_status pd_scan_get_receptor_grids(pd_scan* h, double* grids) {
+  if (!h || !grids) return fail(PD_ERR_INVALID_ARGUMENT, "null argument");
+  if (!h->receptor_ready) return fail(PD_ERR_STATE, "receptor not set");
+  PD_HIP(hipSetDevice(h->d.device));
+  PD_HIP(hipMemcpy(grids, h->R64, sizeof(double) * h->N * h->g.T, hipMemcpyDeviceToHost));
+  return PD_OK;
+}
+
+pd_status pd_scan_set_ligand(pd_scan* h, int n_atoms, const double* pos, const double* weights) {
+  if (!h || !pos || !weights || n_atoms < 1) return fail(PD_ERR_INVALID_ARGUMENT, "bad ligand arguments");
+  PD_HIP(hipSetDevice(h->d.device));
+  ScanGeom& g = h->g;
+  // Rotation-invariant box of the ligand: every rotated atom lies within
+  // r_max of the origin, so its cells lie in [cell(-r), cell(r)] per axis.
+  double rmax = 0;
+  for (int j = 0; j < n_atoms; ++j) {
+    const double* x = pos + 3 * j;
+    double r2 = x[0] * x[0] + x[1] * x[1] + x[2] * x[2];
+    if (!(r2 == r2) || std::isinf(r2)) return fail(PD_ERR_INVALID_ARGUMENT, "non-finite ligand coordinate");
+    rmax = std::max(rmax, std::sqrt(r2));
+  }
+  const double r = rmax * (1.0 + 1e-12) + 1e-9;
+  for (int a = 0; a < 3; ++a) {
+    int lo = to_logical_h(-r, g.zero_h[a], g.h), hi = to_logical_h(r, g.zero_h[a], g.h);
+    lo = std::max(0, lo);
+    hi = std::min(h->d.n[a] - 1, hi);
+    if (hi < lo) return fail(PD_ERR_DOMAIN, "ligand does not overlap the grid");
+    g.lo[a] = lo;
+    g.p[a] = hi - lo + 1;
+  }
+  if (!plan_fits(h->hz, g.p[1], kPlaneThreads))
+    return fail(PD_ERR_UNSUPPORTED, "ligand box too large for the register plan");
+  hipFree(h->q); hipFree(h->w64); hipFree(h->w32);
+  h->q = nullptr; h->w64 = nullptr; h->w32 = nullptr;
+  free_batch(h);
+  h->ligand_ready = false;
+  const size_t nw = (size_t)g.T * n_atoms;
+  std::vector<float> w32(nw);
+  for (size_t i = 0; i < nw; ++i) w32[i] = (float)weights[i];
+  PD_HIP(hipMalloc(&h->q, sizeof(double) * 3 * n_atoms));
+  PD_HIP(hipMalloc(&h->w64, sizeof(double) * nw));
+  PD_HIP(hipMalloc(&h->w32, sizeof(float) * nw));
+  PD_HIP(hipMemcpy(h->q, pos, sizeof(double) * 3 * n_atoms, hipMemcpyHostToDevice));
+  PD_HIP(hipMemcpy(h->w64, weights, sizeof(double) * nw, hipMemcpyHostToDevice));
+  PD_HIP(hipMemcpy(h->w32, w32.data(), sizeof(float) * nw, hipMemcpyHostToDevice));
+  h->n_lig = n_atoms;
+  const size_t B = h->B;
+  PD_HIP(hipMalloc(&h->cells, sizeof(int) * B * n_atoms));
+  PD_HIP(hipMalloc(&h->L1, sizeof(float2) * B * g.T * (size_t)g.p[0] * g.M));
+  PD_HIP(hipMalloc(&h->S1, sizeof(float2) * B * (size_t)g.n0 * g.M));
+  PD_HIP(hipMalloc(&h->cval, sizeof(float) * B * g.n0 * h->K));
+  PD_HIP(hipMalloc(&h->coff, sizeof(int) * B * g.n0 * h->K));
+  h->ligand_ready = true;
+  return PD_OK;
+}
+
+pd_status pd_scan_run_device(pd_scan* h, int n_rot, const double* d_rots, int rot_index0,
+                             pd_peak* d_peaks, int* d_n_out, int sync) {
+  if (!h || (n_rot > 0 && (!d_rots || !d_peaks || !d_n_out)))
+    return fail(PD_ERR_INVALID_ARGUMENT, "null argument");
+  PD_HIP(hipSetDevice(h->d.device));
+  pd_status st = run_batches(h, n_rot, d_rots, rot_index0, reinterpret_cast<PeakOut*>(d_peaks), d_n_out);
+  if (st != PD_OK) return st;
+  if (sync) return pd_scan_sync(h);
+  return PD_OK;
+}
+
+pd_status pd_scan_sync(pd_scan* h) {
+  if (!h) return fail(PD_ERR_INVALID_ARGUMENT, "null handle");
+  PD_HIP(hipSetDevice(h->d.device));
+  pd_status st = resolve_timing(h);
+  if (st != PD_OK) return st;
+  return check_err(h);
+}
+
+pd_status pd_scan_run(pd_scan* h, int n_rot, const double* rots, int rot_index0, pd_peak* peaks,
+                      int* n_out) {
+  if (!h || (n_rot > 0 && (!rots || !peaks || !n_out))) return fail(PD_ERR_INVALID_ARGUMENT, "null argument");
+  if (n_rot == 0) return PD_OK;
+  PD_HIP(hipSetDevice(h->d.device));
+  pd_status st = ensure_io(h, n_rot);
+  if (st != PD_OK) return st;
+  PD_HIP(hipMemcpyAsync(h->d_rots, rots, sizeof(double) * 9 * (size_t)n_rot, hipMemcpyHostToDevice,
+                        h->stream));
+  st = run_batches(h, n_rot, h->d_rots, rot_index0, h->d_peaks, h->d_nout);
+  if (st != PD_OK) return st;
+  PD_HIP(hipMemcpyAsync(peaks, h->d_peaks, sizeof(PeakOut) * (size_t)n_rot * h->K,
+                        hipMemcpyDeviceToHost, h->stream));
+  PD_HIP(hipMemcpyAsync(n_out, h->d_nout, sizeof(int) * (size_t)n_rot, hipMemcpyDeviceToHost, h->stream));
+  return pd_scan_sync(h);
+}
+
+pd_status pd_scan_rescore64(pd_scan* h, int n, const double* rots, const int* s, double* energies) {
+  if (!h || (n > 0 && (!rots || !s || !energies))) return fail(PD_ERR_INVALID_ARGUMENT, "null argument");
+  if (!h->receptor_ready || !h->ligand_ready) return fail(PD_ERR_STATE, "receptor or ligand not set");
+  if (n == 0) return PD_OK;
+  PD_HIP(hipSetDevice(h->d.device));
+  double *dr = nullptr, *de = nullptr;
+  int* ds = nullptr;
+  PD_HIP(hipMalloc(&dr, sizeof(double) * 9 * (size_t)n));
+  PD_HIP(hipMalloc(&ds, sizeof(int) * 3 * (size_t)n));
+  PD_HIP(hipMalloc(&de, sizeof(double) * (size_t)n));
+  PD_HIP(hipMemcpy(dr, rots, sizeof(double) * 9 * (size_t)n, hipMemcpyHostToDevice));
+  PD_HIP(hipMemcpy(ds, s, sizeof(int) * 3 * (size_t)n, hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_rescore, dim3(n), dim3(256), 0, h->stream, h->g, dr, ds, h->q, h->n_lig, h->w64,
+                     h->R64, de);
+  PD_HIP(hipGetLastError());
+  PD_HIP(hipMemcpyAsync(energies, de, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost, h->stream));
+  PD_HIP(hipStreamSynchronize(h->stream));
+  hipFree(dr);
+  hipFree(ds);
+  hipFree(de);
+  return PD_OK;
+}
+
+pd_status pd_scan_correlation_grid(pd_scan* h, const double* rot, float* out) {
+  if (!h || !rot || !out) return fail(PD_ERR_INVALID_ARGUMENT, "null argument");
+  if (!h->receptor_ready || !h->ligand_ready) return fail(PD_ERR_STATE, "receptor or ligand not set");
+  PD_HIP(hipSetDevice(h->d.device));
+  pd_status st = ensure_io(h, 1);
+  if (st != PD_OK) return st;
+  float* dgrid = nullptr;
+  PD_HIP(hipMalloc(&dgrid, sizeof(float) * h->N));
+  PD_HIP(hipMemcpy(h->d_rots, rot, sizeof(double) * 9, hipMemcpyHostToDevice));
+  const ScanGeom& g = h->g;
+  const int ntile = (g.M + h->C - 1) / h->C;
+  hipLaunchKernelGGL(k_bin, dim3((h->n_lig + 255) / 256), dim3(256), 0, h->stream, g, h->d_rots, 1, h->q,
+                     h->n_lig, h->cells, h->err);
+  hipLaunchKernelGGL(k_lig_plane_fwd, dim3(g.p[0], 1), dim3(kPlaneThreads), plane_lds(h), h->stream, g,
+                     h->pz, h->py, h->tw2, h->cells, h->n_lig, h->w32, h->L1);
+  hipLaunchKernelGGL(k_col, dim3(ntile, 1), dim3(kColThreads), col_lds(h), h->stream, g, h->px, h->L1,
+                     h->Rhat, h->S1, h->C, 1, 1);
+  hipLaunchKernelGGL(k_plane_inv_topk, dim3(g.n0, 1), dim3(kPlaneThreads), plane_lds(h), h->stream, g,
+                     h->pz, h->py, h->tw2, h->S1, (float)(1.0 / (double)h->N), h->K, h->cval, h->coff,
+                     dgrid);
+  PD_HIP(hipGetLastError());
+  PD_HIP(hipMemcpyAsync(out, dgrid, sizeof(float) * h->N, hipMemcpyDeviceToHost, h->stream));
+  PD_HIP(hipStreamSynchronize(h->stream));
+  hipFree(dgrid);
+  return check_err(h);
+}
+
+pd_status pd_scan_set_timing(pd_scan* h, int enable) {
+  if (!h) return fail(PD_ERR_INVALID_ARGUMENT, "null handle");
+  pd_status st = resolve_timing(h);
+  if (st != PD_OK) return st;
+  h->timing = enable != 0;
+  for (int k = 0; k < 5; ++k) { h->tot_ms[k] = 0; h->launches[k] = 0; }
+  return PD_OK;
+}
+
+pd_status pd_scan_kernel_stats(pd_scan* h, int k, double* total_ms, long long* launches) {
+  if (!h || k < 0 || k >= 5 || !total_ms || !launches) return fail(PD_ERR_INVALID_ARGUMENT, "bad argument");
+  pd_status st = resolve_timing(h);
+  if (st != PD_OK) return st;
+  *total_ms = h->tot_ms[k];
+  *launches = h->launches[k];
+  return PD_OK;
+}
+
+void* pd_scan_stream(pd_scan* h) { return h ? (void*)h->stream : nullptr; }
+
+}  // extern "C"

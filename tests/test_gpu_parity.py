@@ -1,0 +1,161 @@
+"""GPU parity: libpdfft.so (through its C ABI) against the FP64 CPU oracle.
+
+Bars:
+  * receptor projection (FP64 on the GPU): bit-identical to the oracle;
+  * correlation values (FP32 pipeline): |gpu - oracle| <= tol, with
+    tol = TOL_REL * sum_t ||w_t||_2 * ||R_t||_2 / sqrt(N0 N1 N2) * sqrt(N0 N1 N2)/... (see tolerance());
+  * selected translations (integers): identical to the oracle's at every rank
+    whose oracle value is separated from its neighbours by more than 2*tol;
+  * FP64 rescoring: within 1e-9 relative of the oracle's direct sum.
+"""
+import numpy as np
+import pytest
+
+from tests.conftest import gpu_available
+
+pytestmark = pytest.mark.gpu
+
+if not gpu_available():  # collected on CPU too, but every test needs the device
+    pytest.skip("no GPU", allow_module_level=True)
+
+from proddl2_amd import pdfft, synth  # noqa: E402
+
+# FP32 error model: an FFT correlation's rounding error per output value is
+# bounded by ~ eps32 * log2(N) * ||L||_2 * ||R||_2 (Cauchy-Schwarz on the
+# Parseval sum); TOL_REL absorbs the constant.
+EPS32 = float(np.finfo(np.float32).eps)
+
+
+def tolerance(rec_grids, w, n):
+    N = int(np.prod(n))
+    lg = np.log2(N)
+    return 4.0 * EPS32 * lg * sum(np.linalg.norm(w[t]) * np.linalg.norm(rec_grids[t]) for t in range(len(w)))
+
+
+def build_case(oracle, n, h, n_rec, n_lig, n_terms, seed):
+    cx = synth.make_complex(n_rec, n_lig, seed)
+    rt, w = synth.terms_for(cx, n_terms)
+    R = np.stack([oracle.project_receptor(n, h, cx.receptor.pos, k, p, cut, md) for (k, cut, md, p) in rt])
+    return cx, rt, w, R
+
+
+def make_scan(n, h, T, K, cx, rt, w, batch=16, max_val=1e16):
+    sc = pdfft.Scan(n, h, T, max_n_trans=K, max_val_corr=max_val, batch=batch)
+    sc.project_receptor(cx.receptor.pos, rt)
+    sc.set_ligand(cx.ligand.pos, w)
+    return sc
+
+
+CASES = [
+    # grid, h, n_rec, n_lig, terms
+    ((32, 32, 32), 1.5, 250, 60, 2),
+    ((40, 36, 48), 1.5, 300, 80, 4),
+    ((64, 64, 64), 1.2, 1800, 450, 2),
+]
+
+
+@pytest.mark.parametrize("n,h,n_rec,n_lig,T", CASES)
+def test_receptor_projection_bit_identical(oracle, n, h, n_rec, n_lig, T):
+    print(f"case receptor projection n={n} T={T}", flush=True)
+    cx, rt, w, R = build_case(oracle, n, h, n_rec, n_lig, T, seed=11)
+    with make_scan(n, h, T, 4, cx, rt, w) as sc:
+        G = sc.receptor_grids()
+    assert G.shape == R.shape
+    bad = np.argwhere(G != R)
+    assert bad.shape[0] == 0, f"{bad.shape[0]} cells differ, first {bad[:3]}: {G[tuple(bad[0])]} vs {R[tuple(bad[0])]}"
+
+
+@pytest.mark.parametrize("n,h,n_rec,n_lig,T", CASES)
+def test_correlation_grid(oracle, n, h, n_rec, n_lig, T):
+    print(f"case correlation grid n={n} T={T}", flush=True)
+    cx, rt, w, R = build_case(oracle, n, h, n_rec, n_lig, T, seed=12)
+    S = oracle.receptor_spectra(R)
+    rots = synth.rotation_set(3, seed=3)
+    tol = tolerance(R, w, n)
+    with make_scan(n, h, T, 4, cx, rt, w) as sc:
+        for r in rots:
+            q = oracle.rotate_points(r, cx.ligand.pos)
+            L = np.stack([oracle.project_ligand(n, h, q, w[t]) for t in range(T)])
+            ref = oracle.correlate(S, L)
+            got = sc.correlation_grid(r).astype(np.float64)
+            err = np.abs(got - ref).max()
+            print(f"  max|err|={err:.3e} tol={tol:.3e} range=[{ref.min():.3e},{ref.max():.3e}]", flush=True)
+            assert err <= tol
+
+
+@pytest.mark.parametrize("n,h,n_rec,n_lig,T", CASES)
+def test_scan_peaks_match_oracle(oracle, n, h, n_rec, n_lig, T):
+    print(f"case scan peaks n={n} T={T}", flush=True)
+    K = 10
+    cx, rt, w, R = build_case(oracle, n, h, n_rec, n_lig, T, seed=13)
+    S = oracle.receptor_spectra(R)
+    rots = synth.rotation_set(12, seed=4)
+    tol = tolerance(R, w, n)
+    nout_o, off_o, s_o, val_o = oracle.scan(n, h, S, cx.ligand.pos, w, rots, K=K + 1, max_val=1e16, n_threads=4)
+    with make_scan(n, h, T, K, cx, rt, w, batch=5) as sc:
+        peaks, nout = sc.run(rots, rot_index0=100)
+    assert (nout == K).all()
+    checked = 0
+    for r in range(len(rots)):
+        v = val_o[r]
+        assert (peaks["rot"][r] == 100 + r).all()
+        assert (peaks["rank"][r] == np.arange(K)).all()
+        for i in range(K):
+            lo_gap = v[i] - v[i - 1] if i > 0 else np.inf
+            hi_gap = v[i + 1] - v[i]
+            if lo_gap > 2 * tol and hi_gap > 2 * tol:
+                assert peaks["offset"][r, i] == off_o[r, i], (r, i, peaks[r], off_o[r], v)
+                assert (peaks["s"][r, i] == s_o[r, i]).all()
+                checked += 1
+            assert abs(peaks["value"][r, i] - v[i]) <= tol + 2 * tol
+    print(f"  checked {checked} of {len(rots) * K} ranks exactly (tol={tol:.3e})", flush=True)
+    assert checked >= len(rots) * K // 2
+
+
+def test_rescore64_matches_direct_sum(oracle):
+    n, h, T = (40, 36, 48), 1.5, 4
+    cx, rt, w, R = build_case(oracle, n, h, 300, 80, T, seed=14)
+    rots = synth.rotation_set(6, seed=5)
+    rng = np.random.default_rng(0)
+    s = rng.integers(-10, 10, size=(6, 3)).astype(np.int32)
+    with make_scan(n, h, T, 4, cx, rt, w) as sc:
+        e = sc.rescore64(rots, s)
+    for i in range(6):
+        q = oracle.rotate_points(rots[i], cx.ligand.pos)
+        ref = oracle.direct_energy(R, oracle.cells(n, h, q), w, s[i])
+        assert abs(e[i] - ref) <= 1e-9 * max(1.0, abs(ref)), (i, e[i], ref)
+
+
+def test_max_val_corr_filter(oracle):
+    n, h, T = (32, 32, 32), 1.5, 2
+    cx, rt, w, R = build_case(oracle, n, h, 250, 60, T, seed=15)
+    S = oracle.receptor_spectra(R)
+    rots = synth.rotation_set(4, seed=6)
+    nout_o, off_o, s_o, val_o = oracle.scan(n, h, S, cx.ligand.pos, w, rots, K=10, max_val=1e16)
+    thr = float(np.median(val_o[:, 5]))
+    with make_scan(n, h, T, 10, cx, rt, w, max_val=thr) as sc:
+        peaks, nout = sc.run(rots)
+    for r in range(len(rots)):
+        assert (peaks["value"][r, : nout[r]] < thr).all()
+        expect = int((val_o[r] < thr).sum())
+        assert abs(int(nout[r]) - expect) <= 1
+
+
+def test_ligand_outside_grid_is_domain_error(oracle):
+    n, h, T = (32, 32, 32), 1.5, 2
+    cx, rt, w, R = build_case(oracle, n, h, 250, 60, T, seed=16)
+    with pdfft.Scan(n, h, T, max_n_trans=4, max_val_corr=1e16) as sc:
+        sc.project_receptor(cx.receptor.pos, rt)
+        far = cx.ligand.pos * 3.0  # radius ~ 22 A > half box 24 A after rotation for some atoms
+        far[0] = [30.0, 0.0, 0.0]
+        sc.set_ligand(far, w)
+        with pytest.raises(pdfft.PdError) as ei:
+            sc.run(synth.rotation_set(2, seed=1))
+        assert ei.value.status == 2
+
+
+def test_call_order_is_state_error():
+    with pdfft.Scan((32, 32, 32), 1.5, 2, max_n_trans=4) as sc:
+        with pytest.raises(pdfft.PdError) as ei:
+            sc.run(synth.rotation_set(1, seed=1))
+        assert ei.value.status == 6
