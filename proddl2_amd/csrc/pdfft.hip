@@ -15,9 +15,37 @@
 #include <vector>
 
 #include "../../include/pdfft.h"
+#include "scan_fast.hpp"
 #include "scan_kernels.hpp"
 
 namespace {
+
+// Kernels specialised on a cubic grid edge (scan_fast.hpp); other grids run
+// the generic runtime-plan kernels of scan_kernels.hpp.
+using ColFn = void (*)(ScanGeom, const float2*, const float2*, const float2*, float2*, int, int);
+using LigFn = void (*)(ScanGeom, const float2*, const float2*, const float2*, const int*, int,
+                       const float*, float2*);
+using InvFn = void (*)(ScanGeom, const float2*, const float2*, const float2*, const float2*, float, int,
+                       float*, int*, float*);
+struct FastSet {
+  int n;
+  int col_lines;  // columns per block of the column kernel
+  ColFn col;
+  LigFn lig;
+  InvFn inv;
+};
+#define PD_FAST(N) \
+  FastSet { N, 256 / Split<N>::B, k_col_fast<N>, k_lig_plane_fwd_fast<N, N>, k_plane_inv_topk_fast<N, N> }
+const FastSet kFast[] = {PD_FAST(32),  PD_FAST(48),  PD_FAST(64),  PD_FAST(96),
+                         PD_FAST(120), PD_FAST(128), PD_FAST(144), PD_FAST(160)};
+#undef PD_FAST
+
+const FastSet* find_fast(int n0, int n1, int n2) {
+  if (n0 != n1 || n1 != n2) return nullptr;
+  for (const FastSet& f : kFast)
+    if (f.n == n0) return &f;
+  return nullptr;
+}
 
 thread_local std::string g_err;
 
@@ -115,6 +143,7 @@ struct pd_scan {
   ScanGeom g{};
   int K = 10, B = 64, C = 32;
   long long N = 0;
+  const FastSet* fast = nullptr;
   HostPlan hx, hy, hz;
   FftPlan px{}, py{}, pz{};
   int4* d_stages = nullptr;  // stage tables of px, py, pz
@@ -232,41 +261,65 @@ pd_status receptor_spectra(pd_scan* h) {
   return PD_OK;
 }
 
+// The four FFT-path kernels of one batch of nb rotations (bin, ligand plane
+// pass, column pass, inverse plane pass + top-K).  grid_out (optional)
+// receives the scaled correlation grids.
+pd_status launch_fft_path(pd_scan* h, int nb, const double* d_rots, float* grid_out) {
+  const ScanGeom& g = h->g;
+  const float scale = (float)(1.0 / (double)h->N);
+  {
+    Timer tm(h, 0);
+    const long long tot = (long long)nb * h->n_lig;
+    hipLaunchKernelGGL(k_bin, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, h->stream, g, d_rots, nb,
+                       h->q, h->n_lig, h->cells, h->err);
+  }
+  const FastSet* f = h->fast;
+  {
+    Timer tm(h, 1);
+    if (f)
+      hipLaunchKernelGGL(f->lig, dim3(g.p[0], nb), dim3(kPlaneThreads), plane_lds(h), h->stream, g,
+                         h->tw_y, h->tw_z, h->tw2, h->cells, h->n_lig, h->w32, h->L1);
+    else
+      hipLaunchKernelGGL(k_lig_plane_fwd, dim3(g.p[0], nb), dim3(kPlaneThreads), plane_lds(h), h->stream,
+                         g, h->pz, h->py, h->tw2, h->cells, h->n_lig, h->w32, h->L1);
+  }
+  {
+    Timer tm(h, 2);
+    const int C = f ? f->col_lines : h->C;
+    const int ntile = (g.M + C - 1) / C;
+    int nyb = std::max(1, std::min(nb, 4096 / std::max(1, ntile)));
+    const int bpb = (nb + nyb - 1) / nyb;
+    nyb = (nb + bpb - 1) / bpb;
+    if (f)
+      hipLaunchKernelGGL(f->col, dim3(ntile, nyb), dim3(kColThreads), 0, h->stream, g, h->tw_x, h->L1,
+                         h->Rhat, h->S1, bpb, nb);
+    else
+      hipLaunchKernelGGL(k_col, dim3(ntile, nyb), dim3(kColThreads), col_lds(h), h->stream, g, h->px,
+                         h->L1, h->Rhat, h->S1, h->C, bpb, nb);
+  }
+  {
+    Timer tm(h, 3);
+    if (f)
+      hipLaunchKernelGGL(f->inv, dim3(g.n0, nb), dim3(kPlaneThreads), plane_lds(h), h->stream, g, h->tw_y,
+                         h->tw_z, h->tw2, h->S1, scale, h->K, h->cval, h->coff, grid_out);
+    else
+      hipLaunchKernelGGL(k_plane_inv_topk, dim3(g.n0, nb), dim3(kPlaneThreads), plane_lds(h), h->stream,
+                         g, h->pz, h->py, h->tw2, h->S1, scale, h->K, h->cval, h->coff, grid_out);
+  }
+  PD_HIP(hipGetLastError());
+  return PD_OK;
+}
+
 pd_status run_batches(pd_scan* h, int n_rot, const double* d_rots, int rot0, PeakOut* d_peaks,
                       int* d_nout) {
   if (!h->receptor_ready) return fail(PD_ERR_STATE, "receptor not set");
   if (!h->ligand_ready) return fail(PD_ERR_STATE, "ligand not set");
   if (n_rot < 0) return fail(PD_ERR_INVALID_ARGUMENT, "n_rot < 0");
   const ScanGeom& g = h->g;
-  const int ntile = (g.M + h->C - 1) / h->C;
-  const float scale = (float)(1.0 / (double)h->N);
   for (int b0 = 0; b0 < n_rot; b0 += h->B) {
     const int nb = std::min(h->B, n_rot - b0);
-    {
-      Timer tm(h, 0);
-      const long long tot = (long long)nb * h->n_lig;
-      hipLaunchKernelGGL(k_bin, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, h->stream, g,
-                         d_rots + 9 * (size_t)b0, nb, h->q, h->n_lig, h->cells, h->err);
-    }
-    {
-      Timer tm(h, 1);
-      hipLaunchKernelGGL(k_lig_plane_fwd, dim3(g.p[0], nb), dim3(kPlaneThreads), plane_lds(h),
-                         h->stream, g, h->pz, h->py, h->tw2, h->cells, h->n_lig, h->w32, h->L1);
-    }
-    {
-      Timer tm(h, 2);
-      int nyb = std::max(1, std::min(nb, 4096 / std::max(1, ntile)));
-      int bpb = (nb + nyb - 1) / nyb;
-      nyb = (nb + bpb - 1) / bpb;
-      hipLaunchKernelGGL(k_col, dim3(ntile, nyb), dim3(kColThreads), col_lds(h), h->stream, g, h->px,
-                         h->L1, h->Rhat, h->S1, h->C, bpb, nb);
-    }
-    {
-      Timer tm(h, 3);
-      hipLaunchKernelGGL(k_plane_inv_topk, dim3(g.n0, nb), dim3(kPlaneThreads), plane_lds(h),
-                         h->stream, g, h->pz, h->py, h->tw2, h->S1, scale, h->K, h->cval, h->coff,
-                         (float*)nullptr);
-    }
+    pd_status st = launch_fft_path(h, nb, d_rots + 9 * (size_t)b0, nullptr);
+    if (st != PD_OK) return st;
     {
       Timer tm(h, 4);
       hipLaunchKernelGGL(k_merge, dim3(nb), dim3(256), (size_t)g.n0 * h->K * 8, h->stream, g, h->K,
@@ -429,6 +482,11 @@ pd_status pd_scan_create(const pd_scan_desc* desc, pd_scan** out) {
   if ((st = set_lds((const void*)k_col, col_lds(h))) != PD_OK) return bad(st, g_err);
   if ((st = set_lds((const void*)k_col_fwd, col_lds(h))) != PD_OK) return bad(st, g_err);
   if ((st = set_lds((const void*)k_merge, g.n0 * h->K * 8)) != PD_OK) return bad(st, g_err);
+  h->fast = find_fast(g.n0, g.n1, g.n2);
+  if (h->fast) {
+    if ((st = set_lds((const void*)h->fast->lig, plane_lds(h))) != PD_OK) return bad(st, g_err);
+    if ((st = set_lds((const void*)h->fast->inv, plane_lds(h))) != PD_OK) return bad(st, g_err);
+  }
   if (hipMalloc(&h->R64, sizeof(double) * h->N * g.T) != hipSuccess ||
       hipMalloc(&h->Rhat, sizeof(float2) * (size_t)g.T * g.n0 * g.M) != hipSuccess ||
       hipMalloc(&h->err, sizeof(int)) != hipSuccess)
@@ -633,18 +691,8 @@ pd_status pd_scan_correlation_grid(pd_scan* h, const double* rot, float* out) {
   float* dgrid = nullptr;
   PD_HIP(hipMalloc(&dgrid, sizeof(float) * h->N));
   PD_HIP(hipMemcpy(h->d_rots, rot, sizeof(double) * 9, hipMemcpyHostToDevice));
-  const ScanGeom& g = h->g;
-  const int ntile = (g.M + h->C - 1) / h->C;
-  hipLaunchKernelGGL(k_bin, dim3((h->n_lig + 255) / 256), dim3(256), 0, h->stream, g, h->d_rots, 1, h->q,
-                     h->n_lig, h->cells, h->err);
-  hipLaunchKernelGGL(k_lig_plane_fwd, dim3(g.p[0], 1), dim3(kPlaneThreads), plane_lds(h), h->stream, g,
-                     h->pz, h->py, h->tw2, h->cells, h->n_lig, h->w32, h->L1);
-  hipLaunchKernelGGL(k_col, dim3(ntile, 1), dim3(kColThreads), col_lds(h), h->stream, g, h->px, h->L1,
-                     h->Rhat, h->S1, h->C, 1, 1);
-  hipLaunchKernelGGL(k_plane_inv_topk, dim3(g.n0, 1), dim3(kPlaneThreads), plane_lds(h), h->stream, g,
-                     h->pz, h->py, h->tw2, h->S1, (float)(1.0 / (double)h->N), h->K, h->cval, h->coff,
-                     dgrid);
-  PD_HIP(hipGetLastError());
+  st = launch_fft_path(h, 1, h->d_rots, dgrid);
+  if (st != PD_OK) return st;
   PD_HIP(hipMemcpyAsync(out, dgrid, sizeof(float) * h->N, hipMemcpyDeviceToHost, h->stream));
   PD_HIP(hipStreamSynchronize(h->stream));
   hipFree(dgrid);

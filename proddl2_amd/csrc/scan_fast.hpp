@@ -1,0 +1,438 @@
+// scan_fast.hpp -- the three hot scan kernels specialised on grid edge.
+//
+// Same dataflow as the generic kernels in scan_kernels.hpp (ligand plane pass,
+// column pass, inverse plane pass + top-K), but every 1D transform is a
+// compile-time four-step FFT held in registers (reg_fft.hpp): B lanes per
+// line, one LDS exchange per transform, inputs loaded straight from HBM into
+// registers and outputs stored straight from registers.
+#pragma once
+#include "reg_fft.hpp"
+#include "scan_kernels.hpp"
+
+// N = A * B with B lanes per line; B = 0 marks an unsupported edge.
+template <int N>
+struct Split {
+  static constexpr int B = (N % 8 == 0 && N / 8 <= 20) ? 8 : (N % 16 == 0 && N / 16 <= 16) ? 16
+                          : (N % 4 == 0 && N / 4 <= 20) ? 4 : 0;
+  static constexpr int A = B ? N / B : 0;
+  static constexpr int J = B ? (A + B - 1) / B : 0;
+};
+
+// X[k] of the real row from the packed half-length spectrum Z (r2c post-step).
+__device__ __forceinline__ float2 r2c_point(float2 zk, float2 zm, float2 w) {
+  const float2 E = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
+  const float2 d = make_float2(zk.x - zm.x, zk.y + zm.y);
+  const float2 O = make_float2(0.5f * d.y, -0.5f * d.x);
+  return cadd(E, cmul(w, O));
+}
+
+// Z[k] for the backward half-length FFT from X[k], X[M-k] (c2r pre-step).
+__device__ __forceinline__ float2 c2r_point(float2 xk, float2 xm, float2 w) {
+  const float2 E2 = make_float2(xk.x + xm.x, xk.y - xm.y);
+  const float2 D = make_float2(xk.x - xm.x, xk.y + xm.y);
+  const float2 O2 = cmulc(D, w);
+  return make_float2(E2.x - O2.y, E2.y + O2.x);
+}
+
+// ---------------------------------------------------------------------------
+// Column pass.  grid (ceil(M/C), ceil(nrot/bpb)), block 256, C = 256/B lines.
+// ---------------------------------------------------------------------------
+template <int N0>
+__global__ __launch_bounds__(256) void k_col_fast(ScanGeom g, const float2* __restrict__ twN0,
+                                                  const float2* __restrict__ L1,
+                                                  const float2* __restrict__ Rhat,
+                                                  float2* __restrict__ S1, int bpb, int nrot) {
+  using S = Split<N0>;
+  constexpr int A = S::A, B = S::B, J = S::J, C = 256 / B;
+  __shared__ float2 xb[N0 * C];
+  const int col = threadIdx.x % C, lg = threadIdx.x / C;
+  const int m0 = blockIdx.x * C;
+  const bool colok = m0 + col < g.M;
+  const int bbeg = blockIdx.y * bpb, bend = min(nrot, bbeg + bpb);
+  for (int b = bbeg; b < bend; ++b) {
+    float2 acc[J][B];
+#pragma unroll
+    for (int j = 0; j < J; ++j)
+#pragma unroll
+      for (int k = 0; k < B; ++k) acc[j][k] = make_float2(0.f, 0.f);
+    for (int t = 0; t < g.T; ++t) {
+      const float2* src = L1 + ((size_t)b * g.T + t) * g.p[0] * (size_t)g.M + m0 + col;
+      float2 v[A];
+#pragma unroll
+      for (int m = 0; m < A; ++m) {
+        const int p = lg + B * m - g.lo[0];
+        v[m] = (colok && p >= 0 && p < g.p[0]) ? src[(size_t)p * g.M] : make_float2(0.f, 0.f);
+      }
+      four_step_phase1<A, B, -1>(v, lg, twN0);
+#pragma unroll
+      for (int k1 = 0; k1 < A; ++k1) xb[(k1 * B + lg) * C + col] = v[k1];
+      __syncthreads();
+      float2 w[J][B];
+#pragma unroll
+      for (int j = 0; j < J; ++j) {
+        const int k1 = lg + B * j;
+        if (k1 < A)
+#pragma unroll
+          for (int q = 0; q < B; ++q) w[j][q] = xb[(k1 * B + q) * C + col];
+      }
+      __syncthreads();
+      const float2* R = Rhat + (size_t)t * g.n0 * g.M + m0 + col;
+#pragma unroll
+      for (int j = 0; j < J; ++j) {
+        const int k1 = lg + B * j;
+        if (k1 < A) {
+          rdft<B, -1>(w[j]);
+          if (colok)
+#pragma unroll
+            for (int k2 = 0; k2 < B; ++k2)
+              acc[j][k2] = cadd(acc[j][k2], cmulc(w[j][k2], R[(size_t)(k1 + A * k2) * g.M]));
+        }
+      }
+    }
+    // inverse along x: DFT_B over k2, twiddle, exchange, DFT_A over k1
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+      const int k1 = lg + B * j;
+      if (k1 < A) {
+        four_step_dif_phase1<A, B, +1>(acc[j], k1, twN0);
+#pragma unroll
+        for (int q = 0; q < B; ++q) xb[(k1 * B + q) * C + col] = acc[j][q];
+      }
+    }
+    __syncthreads();
+    float2 v[A];
+#pragma unroll
+    for (int k1 = 0; k1 < A; ++k1) v[k1] = xb[(k1 * B + lg) * C + col];
+    __syncthreads();
+    rdft<A, +1>(v);
+    if (colok) {
+      float2* dst = S1 + (size_t)b * g.n0 * g.M + m0 + col;
+#pragma unroll
+      for (int m = 0; m < A; ++m) dst[(size_t)(lg + B * m) * g.M] = v[m];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Ligand plane pass.  grid (p0, nrot), block 512, LDS N1 * LS complex.
+// ---------------------------------------------------------------------------
+template <int N1, int N2>
+__global__ __launch_bounds__(512) void k_lig_plane_fwd_fast(ScanGeom g, const float2* __restrict__ twN1,
+                                                            const float2* __restrict__ twM2,
+                                                            const float2* __restrict__ tw2,
+                                                            const int* __restrict__ cells, int n_atoms,
+                                                            const float* __restrict__ w,
+                                                            float2* __restrict__ L1) {
+  constexpr int M2 = N2 / 2, H2 = M2 + 1, LS = H2 | 1, NT = 512;
+  using SY = Split<N1>;
+  using SZ = Split<M2>;
+  constexpr int A1 = SY::A, B1 = SY::B, J1 = SY::J;
+  constexpr int A2 = SZ::A, B2 = SZ::B, J2 = SZ::J;
+  constexpr int LY = NT / B1, RY = (H2 + LY - 1) / LY;
+  constexpr int LZ = NT / B2, RZ = (N1 + LZ - 1) / LZ;
+  extern __shared__ float2 buf[];
+  float* fb = reinterpret_cast<float*>(buf);
+  const int tid = threadIdx.x;
+  const int p = blockIdx.x, b = blockIdx.y;
+  const int i0 = g.lo[0] + p;
+  const int lo1 = g.lo[1], p1 = g.p[1];
+  const int plane = N1 * N2;
+  const int* cb = cells + (size_t)b * n_atoms;
+  const int gz = tid / LZ, rz = tid % LZ, gy = tid / LY, cy = tid % LY;
+  for (int t = 0; t < g.T; ++t) {
+    for (int e = tid; e < p1 * M2; e += NT) buf[(lo1 + e / M2) * LS + e % M2] = make_float2(0.f, 0.f);
+    __syncthreads();
+    const float* wt = w + (size_t)t * n_atoms;
+    for (int j = tid; j < n_atoms; j += NT) {
+      const int off = cb[j];
+      if (off >= 0 && off / plane == i0) {
+        const int r = off - i0 * plane;
+        const int i1 = r / N2, i2 = r - i1 * N2;
+        atomicAdd(&fb[i1 * 2 * LS + i2], wt[j]);
+      }
+    }
+    __syncthreads();
+    // z pass: packed half-length forward FFT of rows lo1 .. lo1+p1-1
+    {
+      float2 z[RZ][A2];
+#pragma unroll
+      for (int r = 0; r < RZ; ++r) {
+        const int rr = r * LZ + rz;
+        if (rr < p1) {
+          const int row = lo1 + rr;
+#pragma unroll
+          for (int m = 0; m < A2; ++m) z[r][m] = buf[row * LS + gz + B2 * m];
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int r = 0; r < RZ; ++r) {
+        const int rr = r * LZ + rz;
+        if (rr < p1) {
+          const int row = lo1 + rr;
+          four_step_phase1<A2, B2, -1>(z[r], gz, twM2);
+#pragma unroll
+          for (int k1 = 0; k1 < A2; ++k1) buf[row * LS + k1 * B2 + gz] = z[r][k1];
+        }
+      }
+      __syncthreads();
+      float2 u[RZ][J2][B2];
+#pragma unroll
+      for (int r = 0; r < RZ; ++r) {
+        const int rr = r * LZ + rz;
+#pragma unroll
+        for (int j = 0; j < J2; ++j) {
+          const int k1 = gz + B2 * j;
+          if (rr < p1 && k1 < A2)
+#pragma unroll
+            for (int q = 0; q < B2; ++q) u[r][j][q] = buf[(lo1 + rr) * LS + k1 * B2 + q];
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int r = 0; r < RZ; ++r) {
+        const int rr = r * LZ + rz;
+#pragma unroll
+        for (int j = 0; j < J2; ++j) {
+          const int k1 = gz + B2 * j;
+          if (rr < p1 && k1 < A2) {
+            rdft<B2, -1>(u[r][j]);
+#pragma unroll
+            for (int k2 = 0; k2 < B2; ++k2) buf[(lo1 + rr) * LS + k1 + A2 * k2] = u[r][j][k2];
+          }
+        }
+      }
+      __syncthreads();
+    }
+    // y pass: r2c post-step on the fly, forward FFT along y, store to HBM
+    {
+      float2 v[RY][A1];
+#pragma unroll
+      for (int r = 0; r < RY; ++r) {
+        const int col = r * LY + cy;
+        if (col < H2) {
+          const float2 wz = tw2[col];
+          const int ck = col % M2, cm = (M2 - col) % M2;
+#pragma unroll
+          for (int m = 0; m < A1; ++m) {
+            const int i1 = gy + B1 * m;
+            const int rr = i1 - lo1;
+            v[r][m] = (rr >= 0 && rr < p1) ? r2c_point(buf[i1 * LS + ck], buf[i1 * LS + cm], wz)
+                                            : make_float2(0.f, 0.f);
+          }
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int r = 0; r < RY; ++r) {
+        const int col = r * LY + cy;
+        if (col < H2) {
+          four_step_phase1<A1, B1, -1>(v[r], gy, twN1);
+#pragma unroll
+          for (int k1 = 0; k1 < A1; ++k1) buf[(k1 * B1 + gy) * LS + col] = v[r][k1];
+        }
+      }
+      __syncthreads();
+      float2* dst = L1 + (((size_t)b * g.T + t) * g.p[0] + p) * (size_t)(N1 * H2);
+#pragma unroll
+      for (int r = 0; r < RY; ++r) {
+        const int col = r * LY + cy;
+#pragma unroll
+        for (int j = 0; j < J1; ++j) {
+          const int k1 = gy + B1 * j;
+          if (col < H2 && k1 < A1) {
+            float2 x[B1];
+#pragma unroll
+            for (int q = 0; q < B1; ++q) x[q] = buf[(k1 * B1 + q) * LS + col];
+            rdft<B1, -1>(x);
+#pragma unroll
+            for (int k2 = 0; k2 < B1; ++k2) dst[(k1 + A1 * k2) * H2 + col] = x[k2];
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Inverse plane pass + per-plane top-K.  grid (n0, nrot), block 512.
+// ---------------------------------------------------------------------------
+template <int N1, int N2>
+__global__ __launch_bounds__(512) void k_plane_inv_topk_fast(ScanGeom g, const float2* __restrict__ twN1,
+                                                             const float2* __restrict__ twM2,
+                                                             const float2* __restrict__ tw2,
+                                                             const float2* __restrict__ S1, float scale,
+                                                             int K, float* __restrict__ cval,
+                                                             int* __restrict__ coff,
+                                                             float* __restrict__ grid_out) {
+  constexpr int M2 = N2 / 2, H2 = M2 + 1, LS = H2 | 1, NT = 512;
+  using SY = Split<N1>;
+  using SZ = Split<M2>;
+  constexpr int A1 = SY::A, B1 = SY::B, J1 = SY::J;
+  constexpr int A2 = SZ::A, B2 = SZ::B, J2 = SZ::J;
+  constexpr int LY = NT / B1, RY = (H2 + LY - 1) / LY;
+  constexpr int LZ = NT / B2, RZ = (N1 + LZ - 1) / LZ;
+  extern __shared__ float2 buf[];
+  __shared__ float red_v[NT / 64];
+  __shared__ int red_o[NT / 64];
+  const int tid = threadIdx.x;
+  const int i0 = blockIdx.x, b = blockIdx.y;
+  const float2* src = S1 + ((size_t)b * g.n0 + i0) * (size_t)(N1 * H2);
+  // y pass: backward FFT along y of every column
+  {
+    const int gy = tid / LY, cy = tid % LY;
+    float2 v[RY][A1];
+#pragma unroll
+    for (int r = 0; r < RY; ++r) {
+      const int col = r * LY + cy;
+      if (col < H2)
+#pragma unroll
+        for (int m = 0; m < A1; ++m) v[r][m] = src[(gy + B1 * m) * H2 + col];
+    }
+#pragma unroll
+    for (int r = 0; r < RY; ++r) {
+      const int col = r * LY + cy;
+      if (col < H2) {
+        four_step_phase1<A1, B1, +1>(v[r], gy, twN1);
+#pragma unroll
+        for (int k1 = 0; k1 < A1; ++k1) buf[(k1 * B1 + gy) * LS + col] = v[r][k1];
+      }
+    }
+    __syncthreads();
+    float2 w[RY][J1][B1];
+#pragma unroll
+    for (int r = 0; r < RY; ++r) {
+      const int col = r * LY + cy;
+#pragma unroll
+      for (int j = 0; j < J1; ++j) {
+        const int k1 = gy + B1 * j;
+        if (col < H2 && k1 < A1)
+#pragma unroll
+          for (int q = 0; q < B1; ++q) w[r][j][q] = buf[(k1 * B1 + q) * LS + col];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < RY; ++r) {
+      const int col = r * LY + cy;
+#pragma unroll
+      for (int j = 0; j < J1; ++j) {
+        const int k1 = gy + B1 * j;
+        if (col < H2 && k1 < A1) {
+          rdft<B1, +1>(w[r][j]);
+#pragma unroll
+          for (int k2 = 0; k2 < B1; ++k2) buf[(k1 + A1 * k2) * LS + col] = w[r][j][k2];
+        }
+      }
+    }
+    __syncthreads();
+  }
+  // z pass: c2r pre-step on the fly, backward half-length FFT of every row
+  const int gz = tid / LZ, rz = tid % LZ;
+  float2 u[RZ][J2][B2];
+  {
+    float2 z[RZ][A2];
+#pragma unroll
+    for (int r = 0; r < RZ; ++r) {
+      const int row = r * LZ + rz;
+      if (row < N1)
+#pragma unroll
+        for (int m = 0; m < A2; ++m) {
+          const int k = gz + B2 * m;
+          float2 xk = buf[row * LS + k], xm = buf[row * LS + M2 - k];
+          if (k == 0) { xk.y = 0.f; xm.y = 0.f; }
+          z[r][m] = c2r_point(xk, xm, tw2[k]);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < RZ; ++r) {
+      const int row = r * LZ + rz;
+      if (row < N1) {
+        four_step_phase1<A2, B2, +1>(z[r], gz, twM2);
+#pragma unroll
+        for (int k1 = 0; k1 < A2; ++k1) buf[row * LS + k1 * B2 + gz] = z[r][k1];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < RZ; ++r) {
+      const int row = r * LZ + rz;
+#pragma unroll
+      for (int j = 0; j < J2; ++j) {
+        const int k1 = gz + B2 * j;
+        if (row < N1 && k1 < A2) {
+#pragma unroll
+          for (int q = 0; q < B2; ++q) u[r][j][q] = buf[row * LS + k1 * B2 + q];
+          rdft<B2, +1>(u[r][j]);
+        }
+      }
+    }
+  }
+  // scale, optional dump, per-plane top-K over the values held in registers
+  const float INF = __int_as_float(0x7f800000);
+  float bv = INF;
+  int be = INT_MAX;
+#pragma unroll
+  for (int r = 0; r < RZ; ++r) {
+    const int row = r * LZ + rz;
+#pragma unroll
+    for (int j = 0; j < J2; ++j) {
+      const int k1 = gz + B2 * j;
+      if (row < N1 && k1 < A2) {
+#pragma unroll
+        for (int k2 = 0; k2 < B2; ++k2) {
+          const int e = row * N2 + 2 * (k1 + A2 * k2);
+          float vx = u[r][j][k2].x * scale, vy = u[r][j][k2].y * scale;
+          if (grid_out) {
+            float* go = grid_out + ((size_t)b * g.n0 + i0) * (size_t)(N1 * N2);
+            go[e] = vx;
+            go[e + 1] = vy;
+          }
+          if (vx != vx) vx = INF;
+          if (vy != vy) vy = INF;
+          u[r][j][k2] = make_float2(vx, vy);
+          if (pd_better(vx, e, bv, be)) { bv = vx; be = e; }
+          if (pd_better(vy, e + 1, bv, be)) { bv = vy; be = e + 1; }
+        }
+      }
+    }
+  }
+  float* ov = cval + ((size_t)b * g.n0 + i0) * K;
+  int* oo = coff + ((size_t)b * g.n0 + i0) * K;
+  const int plane = N1 * N2;
+  for (int rk = 0; rk < K; ++rk) {
+    float v = bv;
+    int o = be;
+    block_argmin(v, o, red_v, red_o);
+    if (tid == 0) {
+      ov[rk] = v;
+      oo[rk] = (o == INT_MAX) ? -1 : i0 * plane + o;
+    }
+    if (o != INT_MAX && o == be) {
+      bv = INF;
+      be = INT_MAX;
+#pragma unroll
+      for (int r = 0; r < RZ; ++r) {
+        const int row = r * LZ + rz;
+#pragma unroll
+        for (int j = 0; j < J2; ++j) {
+          const int k1 = gz + B2 * j;
+          if (row < N1 && k1 < A2) {
+#pragma unroll
+            for (int k2 = 0; k2 < B2; ++k2) {
+              const int e = row * N2 + 2 * (k1 + A2 * k2);
+              float2 x = u[r][j][k2];
+              if (e == o) x.x = INF;
+              if (e + 1 == o) x.y = INF;
+              u[r][j][k2] = x;
+              if (pd_better(x.x, e, bv, be)) { bv = x.x; be = e; }
+              if (pd_better(x.y, e + 1, bv, be)) { bv = x.y; be = e + 1; }
+            }
+          }
+        }
+      }
+    }
+  }
+}

@@ -47,10 +47,13 @@ def make_scan(n, h, T, K, cx, rt, w, batch=16, max_val=1e16):
 
 
 CASES = [
-    # grid, h, n_rec, n_lig, terms
+    # grid, h, n_rec, n_lig, terms.  Cubic edges in {32,48,64,96,120,128,144,160}
+    # run the edge-specialised kernels, the rest the generic runtime-plan kernels.
     ((32, 32, 32), 1.5, 250, 60, 2),
     ((40, 36, 48), 1.5, 300, 80, 4),
     ((64, 64, 64), 1.2, 1800, 450, 2),
+    ((96, 96, 96), 1.2, 1500, 300, 4),
+    ((120, 120, 120), 1.2, 2000, 500, 2),
 ]
 
 
