@@ -9,6 +9,14 @@
 #include "reg_fft.hpp"
 #include "scan_kernels.hpp"
 
+// Minimum waves per SIMD requested from the register allocator.
+#ifndef PD_COL_WAVES
+#define PD_COL_WAVES 1
+#endif
+#ifndef PD_PLANE_WAVES
+#define PD_PLANE_WAVES 1
+#endif
+
 // N = A * B with B lanes per line; B = 0 marks an unsupported edge.
 template <int N>
 struct Split {
@@ -17,6 +25,20 @@ struct Split {
   static constexpr int A = B ? N / B : 0;
   static constexpr int J = B ? (A + B - 1) / B : 0;
 };
+
+// Raw buffer access: a per-wave descriptor (base in SGPRs), a 32-bit per-lane
+// byte offset and a uniform byte offset, so no 64-bit address lives in VGPRs.
+// Offsets must stay below the descriptor's byte count (hardware range check).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t pd_rsrc(const void* p, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ float2 pd_ld2(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+  return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0));
+}
+__device__ __forceinline__ void pd_st2(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff, float2 v) {
+  using I2 = decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0));
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(I2, v), r, (int)voff, (int)soff, 0);
+}
 
 // X[k] of the real row from the packed half-length spectrum Z (r2c post-step).
 __device__ __forceinline__ float2 r2c_point(float2 zk, float2 zm, float2 w) {
@@ -38,7 +60,7 @@ __device__ __forceinline__ float2 c2r_point(float2 xk, float2 xm, float2 w) {
 // Column pass.  grid (ceil(M/C), ceil(nrot/bpb)), block 256, C = 256/B lines.
 // ---------------------------------------------------------------------------
 template <int N0>
-__global__ __launch_bounds__(256) void k_col_fast(ScanGeom g, const float2* __restrict__ twN0,
+__global__ __launch_bounds__(256, PD_COL_WAVES) void k_col_fast(ScanGeom g, const float2* __restrict__ twN0,
                                                   const float2* __restrict__ L1,
                                                   const float2* __restrict__ Rhat,
                                                   float2* __restrict__ S1, int bpb, int nrot) {
@@ -46,22 +68,33 @@ __global__ __launch_bounds__(256) void k_col_fast(ScanGeom g, const float2* __re
   constexpr int A = S::A, B = S::B, J = S::J, C = 256 / B;
   __shared__ float2 xb[N0 * C];
   const int col = threadIdx.x % C, lg = threadIdx.x / C;
+  __builtin_assume(lg >= 0 && lg < B);
   const int m0 = blockIdx.x * C;
-  const bool colok = m0 + col < g.M;
+  const int M = g.M;
+  const bool colok = m0 + col < M;
   const int bbeg = blockIdx.y * bpb, bend = min(nrot, bbeg + bpb);
+  const int p0 = g.p[0], lo0 = g.lo[0];
+  const uint32_t vcol = (uint32_t)col * 8u;
+  // receptor spectra of all terms: T * n0 * M complex
+  const __amdgpu_buffer_rsrc_t rr = pd_rsrc(Rhat, (uint32_t)g.T * g.n0 * M * 8u);
+#pragma unroll 1
   for (int b = bbeg; b < bend; ++b) {
     float2 acc[J][B];
 #pragma unroll
     for (int j = 0; j < J; ++j)
 #pragma unroll
       for (int k = 0; k < B; ++k) acc[j][k] = make_float2(0.f, 0.f);
+    // this rotation's ligand planes: T * p0 * M complex
+    const __amdgpu_buffer_rsrc_t rl = pd_rsrc(L1 + (size_t)b * g.T * p0 * M, (uint32_t)g.T * p0 * M * 8u);
+#pragma unroll 1
     for (int t = 0; t < g.T; ++t) {
-      const float2* src = L1 + ((size_t)b * g.T + t) * g.p[0] * (size_t)g.M + m0 + col;
+      const uint32_t st = ((uint32_t)t * p0 * M + m0) * 8u;
       float2 v[A];
 #pragma unroll
       for (int m = 0; m < A; ++m) {
-        const int p = lg + B * m - g.lo[0];
-        v[m] = (colok && p >= 0 && p < g.p[0]) ? src[(size_t)p * g.M] : make_float2(0.f, 0.f);
+        const int p = lg + B * m - lo0;
+        v[m] = (colok && p >= 0 && p < p0) ? pd_ld2(rl, (uint32_t)p * M * 8u + vcol, st)
+                                           : make_float2(0.f, 0.f);
       }
       four_step_phase1<A, B, -1>(v, lg, twN0);
 #pragma unroll
@@ -76,16 +109,19 @@ __global__ __launch_bounds__(256) void k_col_fast(ScanGeom g, const float2* __re
           for (int q = 0; q < B; ++q) w[j][q] = xb[(k1 * B + q) * C + col];
       }
       __syncthreads();
-      const float2* R = Rhat + (size_t)t * g.n0 * g.M + m0 + col;
 #pragma unroll
       for (int j = 0; j < J; ++j) {
         const int k1 = lg + B * j;
         if (k1 < A) {
           rdft<B, -1>(w[j]);
-          if (colok)
+          if (colok) {
+            const uint32_t vr = (uint32_t)k1 * M * 8u + vcol;
 #pragma unroll
-            for (int k2 = 0; k2 < B; ++k2)
-              acc[j][k2] = cadd(acc[j][k2], cmulc(w[j][k2], R[(size_t)(k1 + A * k2) * g.M]));
+            for (int k2 = 0; k2 < B; ++k2) {
+              const uint32_t sr = ((uint32_t)(t * g.n0 + A * k2) * M + m0) * 8u;
+              acc[j][k2] = cadd(acc[j][k2], cmulc(w[j][k2], pd_ld2(rr, vr, sr)));
+            }
+          }
         }
       }
     }
@@ -106,9 +142,9 @@ __global__ __launch_bounds__(256) void k_col_fast(ScanGeom g, const float2* __re
     __syncthreads();
     rdft<A, +1>(v);
     if (colok) {
-      float2* dst = S1 + (size_t)b * g.n0 * g.M + m0 + col;
+      const __amdgpu_buffer_rsrc_t rs = pd_rsrc(S1 + (size_t)b * g.n0 * M, (uint32_t)g.n0 * M * 8u);
 #pragma unroll
-      for (int m = 0; m < A; ++m) dst[(size_t)(lg + B * m) * g.M] = v[m];
+      for (int m = 0; m < A; ++m) pd_st2(rs, (uint32_t)(lg + B * m) * M * 8u + vcol, (uint32_t)m0 * 8u, v[m]);
     }
   }
 }
@@ -117,7 +153,7 @@ __global__ __launch_bounds__(256) void k_col_fast(ScanGeom g, const float2* __re
 // Ligand plane pass.  grid (p0, nrot), block 512, LDS N1 * LS complex.
 // ---------------------------------------------------------------------------
 template <int N1, int N2>
-__global__ __launch_bounds__(512) void k_lig_plane_fwd_fast(ScanGeom g, const float2* __restrict__ twN1,
+__global__ __launch_bounds__(512, PD_PLANE_WAVES) void k_lig_plane_fwd_fast(ScanGeom g, const float2* __restrict__ twN1,
                                                             const float2* __restrict__ twM2,
                                                             const float2* __restrict__ tw2,
                                                             const int* __restrict__ cells, int n_atoms,
@@ -259,7 +295,7 @@ __global__ __launch_bounds__(512) void k_lig_plane_fwd_fast(ScanGeom g, const fl
 // Inverse plane pass + per-plane top-K.  grid (n0, nrot), block 512.
 // ---------------------------------------------------------------------------
 template <int N1, int N2>
-__global__ __launch_bounds__(512) void k_plane_inv_topk_fast(ScanGeom g, const float2* __restrict__ twN1,
+__global__ __launch_bounds__(512, PD_PLANE_WAVES) void k_plane_inv_topk_fast(ScanGeom g, const float2* __restrict__ twN1,
                                                              const float2* __restrict__ twM2,
                                                              const float2* __restrict__ tw2,
                                                              const float2* __restrict__ S1, float scale,
