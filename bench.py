@@ -68,6 +68,7 @@ def main() -> int:
     ap.add_argument("--workload", default="exhaustive-128")
     ap.add_argument("--rotations", type=int, default=0, help="override the workload's rotation count")
     ap.add_argument("--batch", type=int, default=64)
+    ap.add_argument("--top-k", type=int, default=0, help="override peaks kept per rotation (maxNTrans)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--seed", type=int, default=2024)
@@ -87,6 +88,8 @@ def main() -> int:
     wl = dict(synth.WORKLOADS[args.workload])
     if args.rotations:
         wl["n_rot"] = args.rotations
+    if args.top_k:
+        wl["K"] = args.top_k
     n, h, T, K, n_rot = wl["n"], wl["h"], wl["n_terms"], wl["K"], wl["n_rot"]
     log(f"workload {args.workload}: grid {n} h={h} T={T} rotations={n_rot} K={K}")
     t0 = time.perf_counter()

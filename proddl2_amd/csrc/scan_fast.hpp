@@ -40,6 +40,70 @@ __device__ __forceinline__ void pd_st2(__amdgpu_buffer_rsrc_t r, uint32_t voff, 
   __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(I2, v), r, (int)voff, (int)soff, 0);
 }
 
+// ---------------------------------------------------------------------------
+// Top-K helpers.  A candidate is a 64-bit key (order-preserving bits of the
+// FP32 value, offset): unsigned key order is the (value, offset) order, i.e.
+// lower energy first and, among equal energies, the earlier row-major cell
+// (the cell the reference's bounded heap keeps, Common/queue.hpp:233-239).
+// ---------------------------------------------------------------------------
+constexpr uint64_t kNoKey = ~0ull;
+
+__device__ __forceinline__ uint64_t pd_key(float v, int off) {
+  uint32_t b = __float_as_uint(v);
+  b = (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+  return ((uint64_t)b << 32) | (uint32_t)off;
+}
+__device__ __forceinline__ float pd_key_value(uint64_t k) {
+  uint32_t b = (uint32_t)(k >> 32);
+  b = (b & 0x80000000u) ? (b & 0x7fffffffu) : ~b;
+  return __uint_as_float(b);
+}
+
+// One DPP step of a wave minimum: lanes whose source lane is out of range
+// keep their own value (old operand = own value).
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ uint64_t dpp_min_step(uint64_t k) {
+  const uint32_t lo = (uint32_t)k, hi = (uint32_t)(k >> 32);
+  const uint32_t l2 = (uint32_t)__builtin_amdgcn_update_dpp((int)lo, (int)lo, CTRL, ROWMASK, 0xf, false);
+  const uint32_t h2 = (uint32_t)__builtin_amdgcn_update_dpp((int)hi, (int)hi, CTRL, ROWMASK, 0xf, false);
+  const uint64_t o = ((uint64_t)h2 << 32) | l2;
+  return o < k ? o : k;
+}
+
+// Minimum of a 64-bit key over the 64 lanes of the wave, returned to every lane:
+// prefix minima within rows (row_shr 1,2,4,8), then row_bcast 15 and 31
+// carry them across rows so lane 63 holds the wave minimum.
+__device__ __forceinline__ uint64_t wave_min_u64(uint64_t k) {
+  k = dpp_min_step<0x111, 0xf>(k);
+  k = dpp_min_step<0x112, 0xf>(k);
+  k = dpp_min_step<0x114, 0xf>(k);
+  k = dpp_min_step<0x118, 0xf>(k);
+  k = dpp_min_step<0x142, 0xa>(k);
+  k = dpp_min_step<0x143, 0xc>(k);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)k, 63);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(k >> 32), 63);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// Insert key into an ascending list of L keys (drops the largest).
+template <int L>
+__device__ __forceinline__ void list_insert(uint64_t (&h)[L], uint64_t k) {
+#pragma unroll
+  for (int i = 0; i < L; ++i) {
+    const uint64_t a = h[i];
+    const bool lt = k < a;
+    h[i] = lt ? k : a;
+    k = lt ? a : k;
+  }
+}
+
+template <int L>
+__device__ __forceinline__ void list_pop(uint64_t (&h)[L]) {
+#pragma unroll
+  for (int i = 0; i + 1 < L; ++i) h[i] = h[i + 1];
+  h[L - 1] = kNoKey;
+}
+
 // X[k] of the real row from the packed half-length spectrum Z (r2c post-step).
 __device__ __forceinline__ float2 r2c_point(float2 zk, float2 zm, float2 w) {
   const float2 E = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
@@ -310,8 +374,6 @@ __global__ __launch_bounds__(512, PD_PLANE_WAVES) void k_plane_inv_topk_fast(Sca
   constexpr int LY = NT / B1, RY = (H2 + LY - 1) / LY;
   constexpr int LZ = NT / B2, RZ = (N1 + LZ - 1) / LZ;
   extern __shared__ float2 buf[];
-  __shared__ float red_v[NT / 64];
-  __shared__ int red_o[NT / 64];
   const int tid = threadIdx.x;
   const int i0 = blockIdx.x, b = blockIdx.y;
   const float2* src = S1 + ((size_t)b * g.n0 + i0) * (size_t)(N1 * H2);
@@ -406,10 +468,14 @@ __global__ __launch_bounds__(512, PD_PLANE_WAVES) void k_plane_inv_topk_fast(Sca
       }
     }
   }
-  // scale, optional dump, per-plane top-K over the values held in registers
+  // scale, optional dump, keys of this lane's values in a sorted list of L
+  constexpr int L = 4;
   const float INF = __int_as_float(0x7f800000);
-  float bv = INF;
-  int be = INT_MAX;
+  const int plane = N1 * N2;
+  const int base = i0 * plane;
+  uint64_t h[L];
+#pragma unroll
+  for (int i = 0; i < L; ++i) h[i] = kNoKey;
 #pragma unroll
   for (int r = 0; r < RZ; ++r) {
     const int row = r * LZ + rz;
@@ -422,33 +488,35 @@ __global__ __launch_bounds__(512, PD_PLANE_WAVES) void k_plane_inv_topk_fast(Sca
           const int e = row * N2 + 2 * (k1 + A2 * k2);
           float vx = u[r][j][k2].x * scale, vy = u[r][j][k2].y * scale;
           if (grid_out) {
-            float* go = grid_out + ((size_t)b * g.n0 + i0) * (size_t)(N1 * N2);
+            float* go = grid_out + ((size_t)b * g.n0 + i0) * (size_t)plane;
             go[e] = vx;
             go[e + 1] = vy;
           }
           if (vx != vx) vx = INF;
           if (vy != vy) vy = INF;
           u[r][j][k2] = make_float2(vx, vy);
-          if (pd_better(vx, e, bv, be)) { bv = vx; be = e; }
-          if (pd_better(vy, e + 1, bv, be)) { bv = vy; be = e + 1; }
+          list_insert(h, pd_key(vx, base + e));
+          list_insert(h, pd_key(vy, base + e + 1));
         }
       }
     }
   }
-  float* ov = cval + ((size_t)b * g.n0 + i0) * K;
-  int* oo = coff + ((size_t)b * g.n0 + i0) * K;
-  const int plane = N1 * N2;
+  // per-wave K best: K rounds of a DPP wave minimum over the list heads; the
+  // lane that supplied the minimum pops it and, once its list is used up,
+  // refills it with its next L keys.
+  __shared__ uint64_t wtop[NT / 64][32];
+  const int lane = tid & 63, wave = tid >> 6;
+  uint64_t last = 0;
+  int popped = 0;
   for (int rk = 0; rk < K; ++rk) {
-    float v = bv;
-    int o = be;
-    block_argmin(v, o, red_v, red_o);
-    if (tid == 0) {
-      ov[rk] = v;
-      oo[rk] = (o == INT_MAX) ? -1 : i0 * plane + o;
+    const uint64_t m = wave_min_u64(h[0]);
+    if (lane == 0) wtop[wave][rk] = m;
+    if (m != kNoKey && h[0] == m) {
+      last = m;
+      list_pop(h);
+      ++popped;
     }
-    if (o != INT_MAX && o == be) {
-      bv = INF;
-      be = INT_MAX;
+    if (h[0] == kNoKey && popped > 0 && popped % L == 0) {
 #pragma unroll
       for (int r = 0; r < RZ; ++r) {
         const int row = r * LZ + rz;
@@ -458,17 +526,37 @@ __global__ __launch_bounds__(512, PD_PLANE_WAVES) void k_plane_inv_topk_fast(Sca
           if (row < N1 && k1 < A2) {
 #pragma unroll
             for (int k2 = 0; k2 < B2; ++k2) {
-              const int e = row * N2 + 2 * (k1 + A2 * k2);
-              float2 x = u[r][j][k2];
-              if (e == o) x.x = INF;
-              if (e + 1 == o) x.y = INF;
-              u[r][j][k2] = x;
-              if (pd_better(x.x, e, bv, be)) { bv = x.x; be = e; }
-              if (pd_better(x.y, e + 1, bv, be)) { bv = x.y; be = e + 1; }
+              const int e = base + row * N2 + 2 * (k1 + A2 * k2);
+              const uint64_t kx = pd_key(u[r][j][k2].x, e), ky = pd_key(u[r][j][k2].y, e + 1);
+              if (kx > last) list_insert(h, kx);
+              if (ky > last) list_insert(h, ky);
             }
           }
         }
       }
+    }
+  }
+  __syncthreads();
+  // block merge by wave 0: NT/64 * K keys, at most 4 per lane
+  if (wave == 0) {
+    constexpr int NW = NT / 64;
+    uint64_t hh[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) hh[i] = kNoKey;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int idx = lane + 64 * q;
+      if (idx < NW * K) list_insert(hh, wtop[idx / K][idx % K]);
+    }
+    float* ov = cval + ((size_t)b * g.n0 + i0) * K;
+    int* oo = coff + ((size_t)b * g.n0 + i0) * K;
+    for (int rk = 0; rk < K; ++rk) {
+      const uint64_t m = wave_min_u64(hh[0]);
+      if (lane == 0) {
+        ov[rk] = m == kNoKey ? INF : pd_key_value(m);
+        oo[rk] = m == kNoKey ? -1 : (int)(uint32_t)m;
+      }
+      if (m != kNoKey && hh[0] == m) list_pop(hh);
     }
   }
 }
