@@ -69,6 +69,9 @@ def main() -> int:
     ap.add_argument("--rotations", type=int, default=0, help="override the workload's rotation count")
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--top-k", type=int, default=0, help="override peaks kept per rotation (maxNTrans)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first copy a 1 GiB device buffer three times (known bytes) so that a counter "
+                         "profile of this run can scale FETCH_SIZE / WRITE_SIZE")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--seed", type=int, default=2024)
@@ -106,6 +109,14 @@ def main() -> int:
     log(f"receptor precompute {precompute_s * 1e3:.1f} ms")
 
     dev = torch.device("cuda:0")
+    if args.copy_calibration:
+        src = torch.ones(1 << 28, dtype=torch.float32, device=dev)  # 1 GiB
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+        log("copy calibration: 3 copies of 1 GiB")
     d_rots = torch.from_numpy(rots).to(dev)
     d_peaks = torch.zeros((n_rot, K * pdfft.PEAK_DTYPE.itemsize // 4), dtype=torch.int32, device=dev)
     d_nout = torch.zeros(n_rot, dtype=torch.int32, device=dev)

@@ -22,7 +22,7 @@ namespace {
 
 // Kernels specialised on a cubic grid edge (scan_fast.hpp); other grids run
 // the generic runtime-plan kernels of scan_kernels.hpp.
-using ColFn = void (*)(ScanGeom, const float2*, const float2*, const float2*, float2*, int, int);
+using ColFn = void (*)(ScanGeom, const float2*, const float2*, const float2*, float2*, int, int, int);
 using LigFn = void (*)(ScanGeom, const float2*, const float2*, const float2*, const int*, int,
                        const float*, float2*);
 using InvFn = void (*)(ScanGeom, const float2*, const float2*, const float2*, const float2*, float, int,
@@ -290,10 +290,11 @@ pd_status launch_fft_path(pd_scan* h, int nb, const double* d_rots, float* grid_
     int nyb = std::max(1, std::min(nb, 4096 / std::max(1, ntile)));
     const int bpb = (nb + nyb - 1) / nyb;
     nyb = (nb + bpb - 1) / bpb;
-    if (f)
-      hipLaunchKernelGGL(f->col, dim3(ntile, nyb), dim3(kColThreads), 0, h->stream, g, h->tw_x, h->L1,
-                         h->Rhat, h->S1, bpb, nb);
-    else
+    if (f) {
+      const int nblk = 8 * ((ntile + 7) / 8) * nyb;
+      hipLaunchKernelGGL(f->col, dim3(nblk), dim3(kColThreads), 0, h->stream, g, h->tw_x, h->L1, h->Rhat,
+                         h->S1, bpb, nb, nyb);
+    } else
       hipLaunchKernelGGL(k_col, dim3(ntile, nyb), dim3(kColThreads), col_lds(h), h->stream, g, h->px,
                          h->L1, h->Rhat, h->S1, h->C, bpb, nb);
   }

@@ -85,6 +85,39 @@ __device__ __forceinline__ uint64_t wave_min_u64(uint64_t k) {
   return ((uint64_t)hi << 32) | lo;
 }
 
+// Minimum of an FP32 value over the wave (same DPP pattern), to every lane.
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ float dpp_fmin_step(float x) {
+  const float y = __int_as_float(
+      __builtin_amdgcn_update_dpp(__float_as_int(x), __float_as_int(x), CTRL, ROWMASK, 0xf, false));
+  return fminf(x, y);
+}
+__device__ __forceinline__ float wave_min_f32(float x) {
+  x = dpp_fmin_step<0x111, 0xf>(x);
+  x = dpp_fmin_step<0x112, 0xf>(x);
+  x = dpp_fmin_step<0x114, 0xf>(x);
+  x = dpp_fmin_step<0x118, 0xf>(x);
+  x = dpp_fmin_step<0x142, 0xa>(x);
+  x = dpp_fmin_step<0x143, 0xc>(x);
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), 63));
+}
+
+// K-th smallest of the 64 lane values x (each lane one value): K rounds of
+// wave minimum, each round retiring the lowest lane holding the minimum.
+// At least K of the wave's values are <= the result.
+__device__ __forceinline__ float wave_kth_smallest(float x, int K) {
+  const int lane = threadIdx.x & 63;
+  float m = x;
+  for (int r = 0; r < K; ++r) {
+    m = wave_min_f32(x);
+    const unsigned long long hit = __ballot(x == m);
+    if (hit == 0ull) break;  // only NaN lanes left
+    if (lane == __ffsll((long long)hit) - 1) x = __int_as_float(0x7f800000);
+    if (m == __int_as_float(0x7f800000)) break;
+  }
+  return m;
+}
+
 // Insert key into an ascending list of L keys (drops the largest).
 template <int L>
 __device__ __forceinline__ void list_insert(uint64_t (&h)[L], uint64_t k) {
@@ -104,6 +137,14 @@ __device__ __forceinline__ void list_pop(uint64_t (&h)[L]) {
   h[L - 1] = kNoKey;
 }
 
+// Orders this wave's LDS accesses: every lane's earlier LDS writes are visible
+// to every lane's later LDS reads (and the reverse), without a block barrier.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // X[k] of the real row from the packed half-length spectrum Z (r2c post-step).
 __device__ __forceinline__ float2 r2c_point(float2 zk, float2 zm, float2 w) {
   const float2 E = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
@@ -121,95 +162,138 @@ __device__ __forceinline__ float2 c2r_point(float2 xk, float2 xm, float2 w) {
 }
 
 // ---------------------------------------------------------------------------
-// Column pass.  grid (ceil(M/C), ceil(nrot/bpb)), block 256, C = 256/B lines.
+// Column pass.  1D grid of 8 * ceil(ntile/8) * nyb blocks, block 256,
+// C = 256/B lines per tile.  Blocks are dealt to the 8 XCDs round-robin, so
+// block id b runs on XCD b % 8; the mapping below gives every rotation group
+// of a column tile the same XCD, and those groups run back to back, so the
+// tile's receptor spectra (T * N0 * C complex) are read once into that XCD's
+// L2 and reused nyb times.
 // ---------------------------------------------------------------------------
 template <int N0>
 __global__ __launch_bounds__(256, PD_COL_WAVES) void k_col_fast(ScanGeom g, const float2* __restrict__ twN0,
                                                   const float2* __restrict__ L1,
                                                   const float2* __restrict__ Rhat,
-                                                  float2* __restrict__ S1, int bpb, int nrot) {
+                                                  float2* __restrict__ S1, int bpb, int nrot, int nyb) {
   using S = Split<N0>;
   constexpr int A = S::A, B = S::B, J = S::J, C = 256 / B;
-  __shared__ float2 xb[N0 * C];
-  const int col = threadIdx.x % C, lg = threadIdx.x / C;
+  // Each wave owns CW = 64/B columns; lane (lg, cl) is lane lg of column cl.
+  // The exchange of a column stays inside its wave: slot (k1, q, cl) at
+  // k1*XS + q*CW + cl, XS padded so each half-wave of a 64-bit access hits
+  // 64 distinct banks.  The twiddle table shares the array (read per use
+  // rather than held in registers).
+  constexpr int CW = 64 / B, XS = B * CW + 8, WX = A * XS;
+  __shared__ float2 xbuf[4 * WX + N0];
+  float2* twl = xbuf + 4 * WX;
+  for (int i = threadIdx.x; i < N0; i += blockDim.x) twl[i] = twN0[i];
+  __syncthreads();
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int lg = lane / CW, cl = lane % CW;
   __builtin_assume(lg >= 0 && lg < B);
-  const int m0 = blockIdx.x * C;
-  const int M = g.M;
+  float2* xb = xbuf + wave * WX;
+  const int col = wave * CW + cl;
+  const int xcd = blockIdx.x & 7, local = blockIdx.x >> 3;
+  const int tile = (local / nyb) * 8 + xcd, rg = local % nyb;
+  const int M = g.M, T = g.T;
+  const int m0 = tile * C;
+  if (m0 >= M) return;
   const bool colok = m0 + col < M;
-  const int bbeg = blockIdx.y * bpb, bend = min(nrot, bbeg + bpb);
+  const int bbeg = rg * bpb, bend = min(nrot, bbeg + bpb);
   const int p0 = g.p[0], lo0 = g.lo[0];
   const uint32_t vcol = (uint32_t)col * 8u;
-  // receptor spectra of all terms: T * n0 * M complex
-  const __amdgpu_buffer_rsrc_t rr = pd_rsrc(Rhat, (uint32_t)g.T * g.n0 * M * 8u);
+  const uint32_t pM8 = (uint32_t)p0 * M * 8u;
+  // receptor spectra of all terms (T * n0 * M) and the batch's ligand planes
+  const __amdgpu_buffer_rsrc_t rr = pd_rsrc(Rhat, (uint32_t)T * g.n0 * M * 8u);
+  const __amdgpu_buffer_rsrc_t rl = pd_rsrc(L1 + (size_t)bbeg * T * p0 * M, (uint32_t)(bend - bbeg) * T * pM8);
+  const __amdgpu_buffer_rsrc_t rs = pd_rsrc(S1 + (size_t)bbeg * g.n0 * M, (uint32_t)(bend - bbeg) * g.n0 * M * 8u);
+  // iteration it = (rotation bbeg + it / T, term it % T); planes of one
+  // iteration start at it * p0 * M complex in rl.
+  auto load_planes = [&](int it, float2 (&x)[A]) {
+    const uint32_t st = (uint32_t)it * pM8 + (uint32_t)m0 * 8u;
+#pragma unroll
+    for (int m = 0; m < A; ++m) {
+      const int p = lg + B * m - lo0;
+      x[m] = (colok && p >= 0 && p < p0) ? pd_ld2(rl, (uint32_t)p * M * 8u + vcol, st) : make_float2(0.f, 0.f);
+    }
+  };
+  const int nit = (bend - bbeg) * T;
+  float2 v[A];
+  if (nit > 0) load_planes(0, v);
+  float2 acc[J][B];
+#pragma unroll
+  for (int j = 0; j < J; ++j)
+#pragma unroll
+    for (int k = 0; k < B; ++k) acc[j][k] = make_float2(0.f, 0.f);
 #pragma unroll 1
-  for (int b = bbeg; b < bend; ++b) {
-    float2 acc[J][B];
+  for (int it = 0; it < nit; ++it) {
+    const int t = it % T;
+    // receptor values of this term, issued before the transform
+    float2 rv[J][B];
 #pragma unroll
-    for (int j = 0; j < J; ++j)
+    for (int j = 0; j < J; ++j) {
+      const int k1 = lg + B * j;
+      if (k1 < A && colok) {
+        const uint32_t vr = (uint32_t)k1 * M * 8u + vcol;
 #pragma unroll
-      for (int k = 0; k < B; ++k) acc[j][k] = make_float2(0.f, 0.f);
-    // this rotation's ligand planes: T * p0 * M complex
-    const __amdgpu_buffer_rsrc_t rl = pd_rsrc(L1 + (size_t)b * g.T * p0 * M, (uint32_t)g.T * p0 * M * 8u);
-#pragma unroll 1
-    for (int t = 0; t < g.T; ++t) {
-      const uint32_t st = ((uint32_t)t * p0 * M + m0) * 8u;
-      float2 v[A];
-#pragma unroll
-      for (int m = 0; m < A; ++m) {
-        const int p = lg + B * m - lo0;
-        v[m] = (colok && p >= 0 && p < p0) ? pd_ld2(rl, (uint32_t)p * M * 8u + vcol, st)
-                                           : make_float2(0.f, 0.f);
-      }
-      four_step_phase1<A, B, -1>(v, lg, twN0);
-#pragma unroll
-      for (int k1 = 0; k1 < A; ++k1) xb[(k1 * B + lg) * C + col] = v[k1];
-      __syncthreads();
-      float2 w[J][B];
-#pragma unroll
-      for (int j = 0; j < J; ++j) {
-        const int k1 = lg + B * j;
-        if (k1 < A)
-#pragma unroll
-          for (int q = 0; q < B; ++q) w[j][q] = xb[(k1 * B + q) * C + col];
-      }
-      __syncthreads();
-#pragma unroll
-      for (int j = 0; j < J; ++j) {
-        const int k1 = lg + B * j;
-        if (k1 < A) {
-          rdft<B, -1>(w[j]);
-          if (colok) {
-            const uint32_t vr = (uint32_t)k1 * M * 8u + vcol;
-#pragma unroll
-            for (int k2 = 0; k2 < B; ++k2) {
-              const uint32_t sr = ((uint32_t)(t * g.n0 + A * k2) * M + m0) * 8u;
-              acc[j][k2] = cadd(acc[j][k2], cmulc(w[j][k2], pd_ld2(rr, vr, sr)));
-            }
-          }
-        }
+        for (int k2 = 0; k2 < B; ++k2)
+          rv[j][k2] = pd_ld2(rr, vr, ((uint32_t)(t * g.n0 + A * k2) * M + m0) * 8u);
       }
     }
-    // inverse along x: DFT_B over k2, twiddle, exchange, DFT_A over k1
+    // next iteration's ligand planes
+    float2 vn[A];
+    if (it + 1 < nit) load_planes(it + 1, vn);
+    four_step_phase1<A, B, -1>(v, lg, twl);
+#pragma unroll
+    for (int k1 = 0; k1 < A; ++k1) xb[k1 * XS + lg * CW + cl] = v[k1];
+    wave_lds_sync();
+    float2 w[J][B];
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+      const int k1 = lg + B * j;
+      if (k1 < A)
+#pragma unroll
+        for (int q = 0; q < B; ++q) w[j][q] = xb[k1 * XS + q * CW + cl];
+    }
+    wave_lds_sync();
 #pragma unroll
     for (int j = 0; j < J; ++j) {
       const int k1 = lg + B * j;
       if (k1 < A) {
-        four_step_dif_phase1<A, B, +1>(acc[j], k1, twN0);
+        rdft<B, -1>(w[j]);
+        if (colok)
 #pragma unroll
-        for (int q = 0; q < B; ++q) xb[(k1 * B + q) * C + col] = acc[j][q];
+          for (int k2 = 0; k2 < B; ++k2) acc[j][k2] = cadd(acc[j][k2], cmulc(w[j][k2], rv[j][k2]));
       }
     }
-    __syncthreads();
-    float2 v[A];
+    if (t == T - 1) {
+      // inverse along x: DFT_B over k2, twiddle, exchange, DFT_A over k1
+      const int b = it / T;
 #pragma unroll
-    for (int k1 = 0; k1 < A; ++k1) v[k1] = xb[(k1 * B + lg) * C + col];
-    __syncthreads();
-    rdft<A, +1>(v);
-    if (colok) {
-      const __amdgpu_buffer_rsrc_t rs = pd_rsrc(S1 + (size_t)b * g.n0 * M, (uint32_t)g.n0 * M * 8u);
+      for (int j = 0; j < J; ++j) {
+        const int k1 = lg + B * j;
+        if (k1 < A) {
+          four_step_dif_phase1<A, B, +1>(acc[j], k1, twl);
 #pragma unroll
-      for (int m = 0; m < A; ++m) pd_st2(rs, (uint32_t)(lg + B * m) * M * 8u + vcol, (uint32_t)m0 * 8u, v[m]);
+          for (int q = 0; q < B; ++q) xb[k1 * XS + q * CW + cl] = acc[j][q];
+        }
+      }
+      wave_lds_sync();
+      float2 x[A];
+#pragma unroll
+      for (int k1 = 0; k1 < A; ++k1) x[k1] = xb[k1 * XS + lg * CW + cl];
+      wave_lds_sync();
+      rdft<A, +1>(x);
+      if (colok) {
+        const uint32_t sb = ((uint32_t)b * g.n0 * M + m0) * 8u;
+#pragma unroll
+        for (int m = 0; m < A; ++m) pd_st2(rs, (uint32_t)(lg + B * m) * M * 8u + vcol, sb, x[m]);
+      }
+#pragma unroll
+      for (int j = 0; j < J; ++j)
+#pragma unroll
+        for (int k = 0; k < B; ++k) acc[j][k] = make_float2(0.f, 0.f);
     }
+#pragma unroll
+    for (int m = 0; m < A; ++m) v[m] = vn[m];
   }
 }
 
@@ -228,7 +312,7 @@ __global__ __launch_bounds__(512, PD_PLANE_WAVES) void k_lig_plane_fwd_fast(Scan
   using SZ = Split<M2>;
   constexpr int A1 = SY::A, B1 = SY::B, J1 = SY::J;
   constexpr int A2 = SZ::A, B2 = SZ::B, J2 = SZ::J;
-  constexpr int LY = NT / B1, RY = (H2 + LY - 1) / LY;
+  constexpr int LY = NT / B1, RY = (M2 + LY - 1) / LY;  // y columns: M2 (0 and M2 packed)
   constexpr int LZ = NT / B2, RZ = (N1 + LZ - 1) / LZ;
   extern __shared__ float2 buf[];
   float* fb = reinterpret_cast<float*>(buf);
@@ -304,21 +388,29 @@ __global__ __launch_bounds__(512, PD_PLANE_WAVES) void k_lig_plane_fwd_fast(Scan
       }
       __syncthreads();
     }
-    // y pass: r2c post-step on the fly, forward FFT along y, store to HBM
+    // y pass: r2c post-step on the fly, forward FFT along y, store to HBM.
+    // Columns 0 and M2 are real along y; they travel packed as one complex
+    // column c = X[.][0] + i X[.][M2] in column slot 0, so the pass covers M2
+    // columns.  Their spectra are split afterwards by Hermitian symmetry.
     {
       float2 v[RY][A1];
 #pragma unroll
       for (int r = 0; r < RY; ++r) {
         const int col = r * LY + cy;
-        if (col < H2) {
+        if (col < M2) {
           const float2 wz = tw2[col];
-          const int ck = col % M2, cm = (M2 - col) % M2;
+          const int cm = (M2 - col) % M2;
 #pragma unroll
           for (int m = 0; m < A1; ++m) {
             const int i1 = gy + B1 * m;
             const int rr = i1 - lo1;
-            v[r][m] = (rr >= 0 && rr < p1) ? r2c_point(buf[i1 * LS + ck], buf[i1 * LS + cm], wz)
-                                            : make_float2(0.f, 0.f);
+            float2 x = make_float2(0.f, 0.f);
+            if (rr >= 0 && rr < p1) {
+              const float2 zk = buf[i1 * LS + col];
+              // column 0: X[0] = Re Z0 + Im Z0, X[M2] = Re Z0 - Im Z0 (both real)
+              x = col == 0 ? make_float2(zk.x + zk.y, zk.x - zk.y) : r2c_point(zk, buf[i1 * LS + cm], wz);
+            }
+            v[r][m] = x;
           }
         }
       }
@@ -326,7 +418,7 @@ __global__ __launch_bounds__(512, PD_PLANE_WAVES) void k_lig_plane_fwd_fast(Scan
 #pragma unroll
       for (int r = 0; r < RY; ++r) {
         const int col = r * LY + cy;
-        if (col < H2) {
+        if (col < M2) {
           four_step_phase1<A1, B1, -1>(v[r], gy, twN1);
 #pragma unroll
           for (int k1 = 0; k1 < A1; ++k1) buf[(k1 * B1 + gy) * LS + col] = v[r][k1];
@@ -340,15 +432,28 @@ __global__ __launch_bounds__(512, PD_PLANE_WAVES) void k_lig_plane_fwd_fast(Scan
 #pragma unroll
         for (int j = 0; j < J1; ++j) {
           const int k1 = gy + B1 * j;
-          if (col < H2 && k1 < A1) {
+          if (col < M2 && k1 < A1) {
             float2 x[B1];
 #pragma unroll
             for (int q = 0; q < B1; ++q) x[q] = buf[(k1 * B1 + q) * LS + col];
             rdft<B1, -1>(x);
+            if (col != 0) {
 #pragma unroll
-            for (int k2 = 0; k2 < B1; ++k2) dst[(k1 + A1 * k2) * H2 + col] = x[k2];
+              for (int k2 = 0; k2 < B1; ++k2) dst[(k1 + A1 * k2) * H2 + col] = x[k2];
+            } else {
+              // packed spectrum C[k] parked in the unused slot column M2
+#pragma unroll
+              for (int k2 = 0; k2 < B1; ++k2) buf[(k1 + A1 * k2) * LS + M2] = x[k2];
+            }
           }
         }
+      }
+      __syncthreads();
+      // Y0[k] = (C[k] + conj C[-k]) / 2, YM[k] = (C[k] - conj C[-k]) / (2i)
+      for (int k = tid; k < N1; k += NT) {
+        const float2 c = buf[k * LS + M2], cm = buf[((N1 - k) % N1) * LS + M2];
+        dst[k * H2] = make_float2(0.5f * (c.x + cm.x), 0.5f * (c.y - cm.y));
+        dst[k * H2 + M2] = make_float2(0.5f * (c.y + cm.y), -0.5f * (c.x - cm.x));
       }
       __syncthreads();
     }
@@ -468,14 +573,12 @@ __global__ __launch_bounds__(512, PD_PLANE_WAVES) void k_plane_inv_topk_fast(Sca
       }
     }
   }
-  // scale, optional dump, keys of this lane's values in a sorted list of L
+  // scale, optional dump, lane minimum
   constexpr int L = 4;
   const float INF = __int_as_float(0x7f800000);
   const int plane = N1 * N2;
   const int base = i0 * plane;
-  uint64_t h[L];
-#pragma unroll
-  for (int i = 0; i < L; ++i) h[i] = kNoKey;
+  float lmin = INF;
 #pragma unroll
   for (int r = 0; r < RZ; ++r) {
     const int row = r * LZ + rz;
@@ -495,8 +598,30 @@ __global__ __launch_bounds__(512, PD_PLANE_WAVES) void k_plane_inv_topk_fast(Sca
           if (vx != vx) vx = INF;
           if (vy != vy) vy = INF;
           u[r][j][k2] = make_float2(vx, vy);
-          list_insert(h, pd_key(vx, base + e));
-          list_insert(h, pd_key(vy, base + e + 1));
+          lmin = fminf(lmin, fminf(vx, vy));
+        }
+      }
+    }
+  }
+  // The wave's K best values are all <= tau, the K-th smallest lane minimum;
+  // only those enter the lane's sorted list of L keys.
+  const float tau = wave_kth_smallest(lmin, K);
+  uint64_t h[L];
+#pragma unroll
+  for (int i = 0; i < L; ++i) h[i] = kNoKey;
+#pragma unroll
+  for (int r = 0; r < RZ; ++r) {
+    const int row = r * LZ + rz;
+#pragma unroll
+    for (int j = 0; j < J2; ++j) {
+      const int k1 = gz + B2 * j;
+      if (row < N1 && k1 < A2) {
+#pragma unroll
+        for (int k2 = 0; k2 < B2; ++k2) {
+          const int e = base + row * N2 + 2 * (k1 + A2 * k2);
+          const float2 x = u[r][j][k2];
+          if (x.x <= tau) list_insert(h, pd_key(x.x, e));
+          if (x.y <= tau) list_insert(h, pd_key(x.y, e + 1));
         }
       }
     }

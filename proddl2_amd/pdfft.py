@@ -88,6 +88,13 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         raise RuntimeError(
             f"{path} is missing: build it with `python -m proddl2_amd.build` "
             "(the scan has no CPU fallback)")
+    # PyTorch wheels carry their own libamdhip64; if torch is present it must be
+    # loaded first so that this library binds to the same HIP runtime (two
+    # runtimes in one process leave torch without devices).
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(path)
     P = C.c_void_p
     i, d = C.c_int, C.c_double
