@@ -15,6 +15,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -35,6 +36,27 @@ def algorithmic_bytes(n, T, B):
         "k_col": T * S + T * S / B + S,           # reads T spectra + receptor/B, writes S
         "k_plane_inv_topk": S,                    # reads S
     }, (2 * T + 2) * S + T * S / B
+
+
+def measured_traffic(workload, batch, kernel):
+    """HBM bytes per launch of ``kernel`` from the committed rocprofv3 counter summary.
+
+    profiles/scan_<workload>_counters.md holds FETCH_SIZE/WRITE_SIZE per dispatch,
+    scaled by what the same counters read for a 1 GiB device copy; its JSON block
+    gives the calibrated bytes per launch.  None when no summary matches this
+    workload and batch size.
+    """
+    path = os.path.join(ROOT, "profiles", f"scan_{workload.replace('-', '')}_counters.md")
+    if not os.path.exists(path):
+        return None
+    text = open(path).read()
+    m = re.search(r"```json\n(.*?)\n```", text, re.S)
+    if not m:
+        return None
+    rec = json.loads(m.group(1))
+    if rec.get("batch") != batch:
+        return None
+    return rec.get("traffic_bytes_per_launch", {}).get(kernel)
 
 
 def cpu_baseline(cx, rt, w, wl, rots, threads, log):
@@ -200,7 +222,7 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(args.workload, args.batch, dom),
             "bytes_per_launch": bytes_per_launch,
             "avg_launch_us": avg_s * 1e6,
         },

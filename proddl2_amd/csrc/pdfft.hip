@@ -35,7 +35,7 @@ struct FastSet {
   InvFn inv;
 };
 #define PD_FAST(N) \
-  FastSet { N, 256 / Split<N>::B, k_col_fast<N>, k_lig_plane_fwd_fast<N, N>, k_plane_inv_topk_fast<N, N> }
+  FastSet { N, 256 / ColSplit<N>::B, k_col_fast<N>, k_lig_plane_fwd_fast<N, N>, k_plane_inv_topk_fast<N, N> }
 const FastSet kFast[] = {PD_FAST(32),  PD_FAST(48),  PD_FAST(64),  PD_FAST(96),
                          PD_FAST(120), PD_FAST(128), PD_FAST(144), PD_FAST(160)};
 #undef PD_FAST

@@ -26,6 +26,22 @@ struct Split {
   static constexpr int J = B ? (A + B - 1) / B : 0;
 };
 
+// Lanes per column of the column pass (16 lanes when PD_COL_B16 and N = 16 A, A >= 4).
+#ifndef PD_COL_B16
+#define PD_COL_B16 0
+#endif
+// Column pass: load the next term's ligand planes while transforming this one.
+#ifndef PD_COL_PREFETCH
+#define PD_COL_PREFETCH 1
+#endif
+template <int N>
+struct ColSplit {
+  static constexpr bool W = PD_COL_B16 && N % 16 == 0 && N / 16 >= 4 && N / 16 <= 16;
+  static constexpr int B = W ? 16 : Split<N>::B;
+  static constexpr int A = W ? N / 16 : Split<N>::A;
+  static constexpr int J = B ? (A + B - 1) / B : 0;
+};
+
 // Raw buffer access: a per-wave descriptor (base in SGPRs), a 32-bit per-lane
 // byte offset and a uniform byte offset, so no 64-bit address lives in VGPRs.
 // Offsets must stay below the descriptor's byte count (hardware range check).
@@ -174,7 +190,7 @@ __global__ __launch_bounds__(256, PD_COL_WAVES) void k_col_fast(ScanGeom g, cons
                                                   const float2* __restrict__ L1,
                                                   const float2* __restrict__ Rhat,
                                                   float2* __restrict__ S1, int bpb, int nrot, int nyb) {
-  using S = Split<N0>;
+  using S = ColSplit<N0>;
   constexpr int A = S::A, B = S::B, J = S::J, C = 256 / B;
   // Each wave owns CW = 64/B columns; lane (lg, cl) is lane lg of column cl.
   // The exchange of a column stays inside its wave: slot (k1, q, cl) at
@@ -217,7 +233,7 @@ __global__ __launch_bounds__(256, PD_COL_WAVES) void k_col_fast(ScanGeom g, cons
   };
   const int nit = (bend - bbeg) * T;
   float2 v[A];
-  if (nit > 0) load_planes(0, v);
+  if (PD_COL_PREFETCH && nit > 0) load_planes(0, v);
   float2 acc[J][B];
 #pragma unroll
   for (int j = 0; j < J; ++j)
@@ -238,9 +254,13 @@ __global__ __launch_bounds__(256, PD_COL_WAVES) void k_col_fast(ScanGeom g, cons
           rv[j][k2] = pd_ld2(rr, vr, ((uint32_t)(t * g.n0 + A * k2) * M + m0) * 8u);
       }
     }
+#if PD_COL_PREFETCH
     // next iteration's ligand planes
     float2 vn[A];
     if (it + 1 < nit) load_planes(it + 1, vn);
+#else
+    load_planes(it, v);
+#endif
     four_step_phase1<A, B, -1>(v, lg, twl);
 #pragma unroll
     for (int k1 = 0; k1 < A; ++k1) xb[k1 * XS + lg * CW + cl] = v[k1];
@@ -292,8 +312,10 @@ __global__ __launch_bounds__(256, PD_COL_WAVES) void k_col_fast(ScanGeom g, cons
 #pragma unroll
         for (int k = 0; k < B; ++k) acc[j][k] = make_float2(0.f, 0.f);
     }
+#if PD_COL_PREFETCH
 #pragma unroll
     for (int m = 0; m < A; ++m) v[m] = vn[m];
+#endif
   }
 }
 
