@@ -56,7 +56,7 @@ typedef struct pd_scan_desc {
   int n[3];               /* FFT grid edges (FFTCorrelator::init, docking_fft.hpp:545-553) */
   double grid_step;       /* option gridStep (docking_app.hpp:72)                           */
   int n_terms;            /* MolForce::nGrids() (docking_molforce.hpp:83)                   */
-  int max_n_trans;        /* option maxNTrans: peaks kept per rotation (docking_app.hpp:84), 1..64 */
+  int max_n_trans;        /* option maxNTrans: peaks kept per rotation (docking_app.hpp:84), 1..32 */
   int max_n_trans_inp;    /* option maxNTransInp (docking_app.hpp:374); >= max_n_trans       */
   double max_val_corr;    /* option maxValCorr (docking_app.hpp:363)                        */
   int batch;              /* rotations per kernel launch (0: 64)                             */
