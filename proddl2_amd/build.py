@@ -34,9 +34,9 @@ def _run(cmd: list[str]) -> None:
 
 def build_lib() -> str:
     os.makedirs(LIB_DIR, exist_ok=True)
-    src = os.path.join(PKG, "csrc", "pdfft.hip")
+    srcs = [os.path.join(PKG, "csrc", f) for f in ("pdfft.hip", "pdprop.hip")]
     tmp = LIB + ".tmp"
-    _run([HIPCC, *HIP_FLAGS, "-shared", "-o", tmp, src])
+    _run([HIPCC, *HIP_FLAGS, "-shared", "-o", tmp, *srcs])
     os.replace(tmp, LIB)
     return LIB
 

@@ -18,6 +18,15 @@
 #include "scan_fast.hpp"
 #include "scan_kernels.hpp"
 
+namespace pd_detail {
+// Message of the calling thread's last failed call, shared by every source of
+// the library (pd_last_error).
+std::string& last_error() {
+  thread_local std::string msg;
+  return msg;
+}
+}  // namespace pd_detail
+
 namespace {
 
 // Kernels specialised on a cubic grid edge (scan_fast.hpp); other grids run
@@ -47,10 +56,8 @@ const FastSet* find_fast(int n0, int n1, int n2) {
   return nullptr;
 }
 
-thread_local std::string g_err;
-
 pd_status fail(pd_status s, const std::string& msg) {
-  g_err = msg;
+  pd_detail::last_error() = msg;
   return s;
 }
 
@@ -363,7 +370,7 @@ pd_status ensure_io(pd_scan* h, int n_rot) {
 
 extern "C" {
 
-const char* pd_last_error(void) { return g_err.c_str(); }
+const char* pd_last_error(void) { return pd_detail::last_error().c_str(); }
 
 int pd_find_best_size(int start) {
   // FFTW_Size::findBestSize (Math/fft.hpp:82-151): first n >= max(2, start)
@@ -455,10 +462,10 @@ pd_status pd_scan_create(const pd_scan_desc* desc, pd_scan** out) {
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
     return bad(PD_ERR_DEVICE, "hipStreamCreate failed");
   pd_status st;
-  if ((st = make_twiddles(h, g.n0, &h->tw_x)) != PD_OK) return bad(st, g_err);
-  if ((st = make_twiddles(h, g.n1, &h->tw_y)) != PD_OK) return bad(st, g_err);
-  if ((st = make_twiddles(h, g.m2, &h->tw_z)) != PD_OK) return bad(st, g_err);
-  if ((st = make_twiddles(h, g.n2, &h->tw2)) != PD_OK) return bad(st, g_err);
+  if ((st = make_twiddles(h, g.n0, &h->tw_x)) != PD_OK) return bad(st, pd_detail::last_error());
+  if ((st = make_twiddles(h, g.n1, &h->tw_y)) != PD_OK) return bad(st, pd_detail::last_error());
+  if ((st = make_twiddles(h, g.m2, &h->tw_z)) != PD_OK) return bad(st, pd_detail::last_error());
+  if ((st = make_twiddles(h, g.n2, &h->tw2)) != PD_OK) return bad(st, pd_detail::last_error());
   if (!make_plan(g.n0, h->hx) || !make_plan(g.n1, h->hy) || !make_plan(g.m2, h->hz))
     return bad(PD_ERR_UNSUPPORTED, "grid edges must factor over {2,3,5,7,11,13}");
   if (!plan_fits(h->hy, g.h2, kPlaneThreads) || !plan_fits(h->hz, g.n1, kPlaneThreads) ||
@@ -477,16 +484,16 @@ pd_status pd_scan_create(const pd_scan_desc* desc, pd_scan** out) {
     h->py = FftPlan{g.n1, ny, h->d_stages + nx, h->tw_y};
     h->pz = FftPlan{g.m2, nz, h->d_stages + nx + ny, h->tw_z};
   }
-  if ((st = set_lds((const void*)k_lig_plane_fwd, plane_lds(h))) != PD_OK) return bad(st, g_err);
-  if ((st = set_lds((const void*)k_plane_inv_topk, plane_lds(h))) != PD_OK) return bad(st, g_err);
-  if ((st = set_lds((const void*)k_rec_plane_fwd, plane_lds(h))) != PD_OK) return bad(st, g_err);
-  if ((st = set_lds((const void*)k_col, col_lds(h))) != PD_OK) return bad(st, g_err);
-  if ((st = set_lds((const void*)k_col_fwd, col_lds(h))) != PD_OK) return bad(st, g_err);
-  if ((st = set_lds((const void*)k_merge, g.n0 * h->K * 8)) != PD_OK) return bad(st, g_err);
+  if ((st = set_lds((const void*)k_lig_plane_fwd, plane_lds(h))) != PD_OK) return bad(st, pd_detail::last_error());
+  if ((st = set_lds((const void*)k_plane_inv_topk, plane_lds(h))) != PD_OK) return bad(st, pd_detail::last_error());
+  if ((st = set_lds((const void*)k_rec_plane_fwd, plane_lds(h))) != PD_OK) return bad(st, pd_detail::last_error());
+  if ((st = set_lds((const void*)k_col, col_lds(h))) != PD_OK) return bad(st, pd_detail::last_error());
+  if ((st = set_lds((const void*)k_col_fwd, col_lds(h))) != PD_OK) return bad(st, pd_detail::last_error());
+  if ((st = set_lds((const void*)k_merge, g.n0 * h->K * 8)) != PD_OK) return bad(st, pd_detail::last_error());
   h->fast = find_fast(g.n0, g.n1, g.n2);
   if (h->fast) {
-    if ((st = set_lds((const void*)h->fast->lig, plane_lds(h))) != PD_OK) return bad(st, g_err);
-    if ((st = set_lds((const void*)h->fast->inv, plane_lds(h))) != PD_OK) return bad(st, g_err);
+    if ((st = set_lds((const void*)h->fast->lig, plane_lds(h))) != PD_OK) return bad(st, pd_detail::last_error());
+    if ((st = set_lds((const void*)h->fast->inv, plane_lds(h))) != PD_OK) return bad(st, pd_detail::last_error());
   }
   if (hipMalloc(&h->R64, sizeof(double) * h->N * g.T) != hipSuccess ||
       hipMalloc(&h->Rhat, sizeof(float2) * (size_t)g.T * g.n0 * g.M) != hipSuccess ||
