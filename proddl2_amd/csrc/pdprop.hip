@@ -509,7 +509,7 @@ pd_status pd_prop_gather_rotations(pd_prop* h, int n, const int* d_idx, const do
 
 }  // extern "C"
 
-namespace {
+namespace prop {
 __global__ void k_remap_peaks(int n, int K, pd_peak* __restrict__ peaks, const int* __restrict__ n_out,
                               const int* __restrict__ idx) {
   const int e = blockIdx.x * 256 + threadIdx.x;
@@ -519,7 +519,7 @@ __global__ void k_remap_peaks(int n, int K, pd_peak* __restrict__ peaks, const i
   const int r = peaks[e].rot;
   if (r >= 0 && r < n) peaks[e].rot = idx[r];
 }
-}  // namespace
+}  // namespace prop
 
 extern "C" pd_status pd_prop_remap_peaks(pd_prop* h, int n, int K, pd_peak* d_peaks, const int* d_n_out,
                                          const int* d_idx) {

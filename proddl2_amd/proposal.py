@@ -40,6 +40,22 @@ TRAIN = dict(n_pairs=200, n_rec=1800, n_lig=450, n_rot=3000, seed0=1000, rot_see
              grid=(128, 128, 128), h=1.2, n_terms=4, epochs=12, lr=1e-3)
 
 
+def _layout():
+    """Offsets of the flat parameter vector (include/pdprop.h, pd_prop_get_params)."""
+    c, cin, hid, nf = 8, 2, 384, 24
+    stack = c * cin * 27 + c + 2 * (c * c * 27 + c)
+    w1 = 2 * stack
+    b1 = w1 + hid * (nf + 6)
+    w2 = b1 + hid
+    b2 = w2 + hid * hid
+    w3 = b2 + hid
+    return dict(conv_stack=stack, w1=w1, b1=b1, w2=w2, b2=b2, w3=w3, b3=w3 + hid, n=w3 + hid + 1,
+                hid=hid, nin=nf + 6, nf=nf)
+
+
+LAYOUT = _layout()
+
+
 class PropDesc(C.Structure):
     _fields_ = [("max_rot", C.c_int), ("device", C.c_int), ("seed", C.c_uint)]
 
@@ -281,6 +297,72 @@ def run_pruned(scan, prop, d_rots, bufs, n_keep: int) -> None:
     prop.remap_peaks_device(n_keep, scan.K, d_peaks.data_ptr(), d_nout.data_ptr(), d_idx.data_ptr())
 
 
+def fixed_score_params() -> np.ndarray:
+    """Parameters under which the network scores a rotation by minus the sum of
+    its 24 standardised rotation functions: identity convolutions (last layer
+    0), no learned weighting.  This is the ranking the learned part improves on."""
+    p = np.zeros(LAYOUT["n"], np.float32)
+    hid, nin, nf = LAYOUT["hid"], LAYOUT["nin"], LAYOUT["nf"]
+    W1 = np.zeros((hid, nin), np.float32)
+    W1[0, :nf] = 1.0
+    W1[1, :nf] = -1.0
+    W2 = np.zeros((hid, hid), np.float32)
+    W2[0, 0], W2[0, 1] = 1.0, -1.0   # relu(s)
+    W2[1, 0], W2[1, 1] = -1.0, 1.0   # relu(-s)
+    W3 = np.zeros(hid, np.float32)
+    W3[0], W3[1] = -1.0, 1.0         # -relu(s) + relu(-s) = -s
+    p[LAYOUT["w1"]:LAYOUT["w1"] + W1.size] = W1.reshape(-1)
+    p[LAYOUT["w2"]:LAYOUT["w2"] + W2.size] = W2.reshape(-1)
+    p[LAYOUT["w3"]:LAYOUT["w3"] + W3.size] = W3
+    return p
+
+
+def pruned_eval(sc, prop: Proposal, d_rots, ex_peaks, t_exhaustive_s: float, keep_fracs=(0.10, 0.15),
+                n_top=100, repeats=3, log=print):
+    """Pruned scans of the rotations d_rots (maps of the complex already set, prop on
+    the scan's stream) against the exhaustive peaks ex_peaks of the same rotations."""
+    torch = _torch()
+    n = d_rots.shape[0]
+    top = top_pose_rotations(ex_peaks, n_top)
+    top_flat = np.argsort(ex_peaks["value"].reshape(-1), kind="stable")[:n_top]
+    ex_pose = set(zip(ex_peaks["rot"].reshape(-1)[top_flat].tolist(),
+                      ex_peaks["offset"].reshape(-1)[top_flat].tolist()))
+    res = {"n_top": n_top, "exhaustive_s": t_exhaustive_s}
+    d_scores = torch.empty(n, dtype=torch.float32, device=d_rots.device)
+    t_pr = []
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        prop.predict_device(n, d_rots.data_ptr(), d_scores.data_ptr())
+        sc.sync()
+        t_pr.append(time.perf_counter() - t0)
+    res["proposal_s"] = float(np.median(t_pr))
+    res["keep"] = []
+    for frac in keep_fracs:
+        n_keep = int(round(frac * n))
+        bufs = pruned_buffers(sc, prop, d_rots, n_keep)
+        t_all = []
+        for _ in range(repeats):
+            t0 = time.perf_counter()
+            run_pruned(sc, prop, d_rots, bufs, n_keep)
+            sc.sync()
+            t_all.append(time.perf_counter() - t0)
+        kept_peaks = bufs[3].cpu().numpy().view(pdfft.PEAK_DTYPE).reshape(n_keep, sc.K)
+        kept = np.zeros(n, bool)
+        kept[bufs[1].cpu().numpy()] = True
+        # the pruned search's own top poses against the exhaustive ones (rotation, cell)
+        pr_flat = np.argsort(kept_peaks["value"].reshape(-1), kind="stable")[:n_top]
+        pr_pose = set(zip(kept_peaks["rot"].reshape(-1)[pr_flat].tolist(),
+                          kept_peaks["offset"].reshape(-1)[pr_flat].tolist()))
+        recall = len(pr_pose & ex_pose) / n_top
+        t = float(np.median(t_all))
+        res["keep"].append({"fraction": frac, "n_keep": n_keep, "recall": recall,
+                            "recall_by_rotation": float(kept[top].mean()), "pruned_s": t,
+                            "speedup": t_exhaustive_s / t})
+        log(f"keep {frac:.2f}: recall@{n_top} {recall:.2f}, pruned {t * 1e3:.1f} ms vs exhaustive "
+            f"{t_exhaustive_s * 1e3:.1f} ms -> {t_exhaustive_s / t:.2f}x")
+    return res
+
+
 def evaluate(prop: Proposal, workload="exhaustive-128", seed=2024, keep_fracs=(0.10, 0.15), n_top=100,
              repeats=3, log=print):
     """Exhaustive vs pruned scan of one complex: recall of the exhaustive top poses and times."""
@@ -291,62 +373,28 @@ def evaluate(prop: Proposal, workload="exhaustive-128", seed=2024, keep_fracs=(0
     rt, w = synth.terms_for(cx, wl["n_terms"])
     rots = synth.rotation_set(wl["n_rot"], seed + 1)
     n = rots.shape[0]
-    res = {"workload": workload, "seed": seed, "rotations": n, "n_top": n_top}
     with pdfft.Scan(wl["n"], wl["h"], wl["n_terms"], max_n_trans=wl["K"], max_val_corr=1e16) as sc:
         sc.project_receptor(cx.receptor.pos, rt)
         sc.set_ligand(cx.ligand.pos, w)
         prop.set_stream(sc.stream())
-        t0 = time.perf_counter()
-        prop.set_complex(cx.receptor.pos, cx.ligand.pos)
-        res["maps_ms"] = (time.perf_counter() - t0) * 1e3
-        d_rots = torch.from_numpy(rots).to(dev)
-        d_peaks = torch.zeros((n, sc.K * pdfft.PEAK_DTYPE.itemsize // 4), dtype=torch.int32, device=dev)
-        d_nout = torch.zeros(n, dtype=torch.int32, device=dev)
-        torch.cuda.synchronize()
-        t_ex = []
-        for _ in range(repeats):
+        try:
             t0 = time.perf_counter()
-            sc.run_device(n, d_rots.data_ptr(), d_peaks.data_ptr(), d_nout.data_ptr(), 0, sync=True)
-            t_ex.append(time.perf_counter() - t0)
-        ex_peaks = d_peaks.cpu().numpy().view(pdfft.PEAK_DTYPE).reshape(n, sc.K)
-        top = top_pose_rotations(ex_peaks, n_top)
-        top_flat = np.argsort(ex_peaks["value"].reshape(-1), kind="stable")[:n_top]
-        res["exhaustive_s"] = float(np.median(t_ex))
-        d_scores = torch.empty(n, dtype=torch.float32, device=dev)
-        t_pr = []
-        for _ in range(repeats):
-            t0 = time.perf_counter()
-            prop.predict_device(n, d_rots.data_ptr(), d_scores.data_ptr())
-            sc.sync()
-            t_pr.append(time.perf_counter() - t0)
-        res["proposal_s"] = float(np.median(t_pr))
-        res["keep"] = []
-        for frac in keep_fracs:
-            n_keep = int(round(frac * n))
-            bufs = pruned_buffers(sc, prop, d_rots, n_keep)
-            t_all = []
+            prop.set_complex(cx.receptor.pos, cx.ligand.pos)
+            maps_ms = (time.perf_counter() - t0) * 1e3
+            d_rots = torch.from_numpy(rots).to(dev)
+            d_peaks = torch.zeros((n, sc.K * pdfft.PEAK_DTYPE.itemsize // 4), dtype=torch.int32, device=dev)
+            d_nout = torch.zeros(n, dtype=torch.int32, device=dev)
+            torch.cuda.synchronize()
+            t_ex = []
             for _ in range(repeats):
                 t0 = time.perf_counter()
-                run_pruned(sc, prop, d_rots, bufs, n_keep)
-                sc.sync()
-                t_all.append(time.perf_counter() - t0)
-            kept_peaks = bufs[3].cpu().numpy().view(pdfft.PEAK_DTYPE).reshape(n_keep, sc.K)
-            kept = np.zeros(n, bool)
-            kept[bufs[1].cpu().numpy()] = True
-            # pruned top poses (same K peaks per kept rotation) against the exhaustive ones
-            pr_flat = np.argsort(kept_peaks["value"].reshape(-1), kind="stable")[:n_top]
-            pr_pose = set(zip(kept_peaks["rot"].reshape(-1)[pr_flat].tolist(),
-                              kept_peaks["offset"].reshape(-1)[pr_flat].tolist()))
-            ex_pose = set(zip(ex_peaks["rot"].reshape(-1)[top_flat].tolist(),
-                              ex_peaks["offset"].reshape(-1)[top_flat].tolist()))
-            recall = len(pr_pose & ex_pose) / n_top
-            t = float(np.median(t_all))
-            res["keep"].append({"fraction": frac, "n_keep": n_keep, "recall": recall,
-                                "recall_by_rotation": float(kept[top].mean()), "pruned_s": t,
-                                "speedup": res["exhaustive_s"] / t})
-            log(f"keep {frac:.2f}: recall@{n_top} {recall:.2f}, pruned {t * 1e3:.1f} ms vs exhaustive "
-                f"{res['exhaustive_s'] * 1e3:.1f} ms -> {res['exhaustive_s'] / t:.2f}x")
-        prop.set_stream(None)
+                sc.run_device(n, d_rots.data_ptr(), d_peaks.data_ptr(), d_nout.data_ptr(), 0, sync=True)
+                t_ex.append(time.perf_counter() - t0)
+            ex_peaks = d_peaks.cpu().numpy().view(pdfft.PEAK_DTYPE).reshape(n, sc.K)
+            res = pruned_eval(sc, prop, d_rots, ex_peaks, float(np.median(t_ex)), keep_fracs, n_top, repeats, log)
+        finally:
+            prop.set_stream(None)
+    res.update({"workload": workload, "seed": seed, "rotations": n, "maps_ms": maps_ms})
     return res
 
 
