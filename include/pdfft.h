@@ -134,6 +134,27 @@ void* pd_scan_stream(struct pd_scan* h);
 void pd_scan_destroy(struct pd_scan* h);
 const char* pd_last_error(void);
 
+/* Rotations per kernel launch (pd_scan_desc.batch), changed after creation;
+ * reallocates the per-batch buffers.  Results do not depend on it. */
+pd_status pd_scan_set_batch(struct pd_scan* h, int batch);
+
+/* Several scans in one captured HIP graph.  No reference counterpart: the
+ * reference runs one rotation range per process (python/lib/PRODDL/dock.py:86-113).
+ * Pair i scans n_rot[i] device rotations d_rots[i] into d_peaks[i] / d_n_out[i]
+ * (rot_index0 = 0) on lane lane[i] < n_lanes; the pairs of a lane run in the given
+ * order, lanes run concurrently.  pd_graph_launch replays everything with one
+ * launch (the buffers named at creation are reused); pd_graph_sync waits and
+ * reports domain errors of any pair.  A scan may appear in one graph at a time
+ * and must not run other work while its graph does. */
+pd_status pd_graph_create(int n, struct pd_scan* const* scans, const int* n_rot,
+                          const double* const* d_rots, pd_peak* const* d_peaks,
+                          int* const* d_n_out, const int* lane, int n_lanes,
+                          struct pd_graph** out);
+pd_status pd_graph_launch(struct pd_graph* g);
+pd_status pd_graph_sync(struct pd_graph* g);
+void* pd_graph_stream(struct pd_graph* g);
+void pd_graph_destroy(struct pd_graph* g);
+
 /* FFTW_Size::findBestSize (Math/fft.hpp:141-151); returns -1 if none. */
 int pd_find_best_size(int start);
 

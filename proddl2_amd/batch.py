@@ -1,0 +1,214 @@
+"""Batched uneven docking pairs in one captured HIP graph, with an autotuner.
+
+Workload: 32 synthetic pairs, receptor 800-6,000 atoms and ligand 200-2,000
+atoms (log-uniform, seeded), 4 terms, grid step drawn per pair from
+{0.8, 1.0, 1.2} A, each on the smallest grid edge of {64, 96, 120, 128, 144,
+160} that holds the receptor, the ligand, the longest term cutoff and two
+cells of margin on both sides (the reference pads the receptor box by
+2h + ligand radius + cutoff per side, include/PRODDL/docking_molstruct.hpp:236);
+5,000 rotations per pair.
+
+The reference runs one rotation range per process (python/lib/PRODDL/dock.py:86-113);
+here every pair keeps its receptor spectra resident and the whole set replays
+as one graph (pd_graph_*).  The autotuner picks, per grid edge, the rotations
+per launch, and then the number of concurrent lanes of the graph, by timing
+them on the device.
+
+    python -m proddl2_amd.batch            # prints one JSON line
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+
+from . import pdfft, synth
+
+EDGES = (64, 96, 120, 128, 144, 160)
+STEPS = (0.8, 1.0, 1.2)
+N_TERMS = 4
+K = 10
+
+
+def radius(n: int) -> float:
+    return (3.0 * n / (4.0 * np.pi * synth.DENSITY)) ** (1.0 / 3.0)
+
+
+def fit_edge(n_rec: int, n_lig: int, h: float) -> int:
+    need = 2.0 * (radius(n_rec) + radius(n_lig) + synth.COULOMB_CUTOFF + 2.0 * h) / h
+    for e in EDGES:
+        if e >= need:
+            return e
+    raise ValueError("pair does not fit the largest edge")
+
+
+def uneven_pairs(n_pairs=32, n_rot=5000, seed=3000):
+    rng = np.random.default_rng(seed)
+    lr = np.exp(rng.uniform(np.log(800), np.log(6000), n_pairs)).astype(int)
+    ll = np.exp(rng.uniform(np.log(200), np.log(2000), n_pairs)).astype(int)
+    hs = rng.choice(np.array(STEPS), n_pairs)
+    out = []
+    for i in range(n_pairs):
+        out.append(dict(n_rec=int(lr[i]), n_lig=int(ll[i]), h=float(hs[i]), edge=fit_edge(lr[i], ll[i], hs[i]),
+                        seed=seed + 1 + i, rot_seed=seed + 1000 + i, n_rot=n_rot))
+    return out
+
+
+class Pair:
+    """One pair: its scan handle, rotations and output buffers on the device."""
+
+    def __init__(self, spec, torch, dev, batch=64):
+        self.spec = spec
+        cx = synth.make_complex(spec["n_rec"], spec["n_lig"], spec["seed"])
+        rt, w = synth.terms_for(cx, N_TERMS)
+        n = spec["edge"]
+        self.scan = pdfft.Scan((n, n, n), spec["h"], N_TERMS, max_n_trans=K, max_val_corr=1e16, batch=batch)
+        self.scan.project_receptor(cx.receptor.pos, rt)
+        self.scan.set_ligand(cx.ligand.pos, w)
+        rots = synth.rotation_set(spec["n_rot"], spec["rot_seed"])
+        self.n_rot = rots.shape[0]
+        self.d_rots = torch.from_numpy(rots).to(dev)
+        words = K * pdfft.PEAK_DTYPE.itemsize // 4
+        self.d_peaks = torch.zeros((self.n_rot, words), dtype=torch.int32, device=dev)
+        self.d_nout = torch.zeros(self.n_rot, dtype=torch.int32, device=dev)
+        self.cost = self.n_rot * n ** 3  # proportional to the FFT bytes
+
+    def run(self, n=None, sync=True):
+        n = self.n_rot if n is None else n
+        self.scan.run_device(n, self.d_rots.data_ptr(), self.d_peaks.data_ptr(), self.d_nout.data_ptr(), 0,
+                             sync=sync)
+
+    def peaks(self):
+        return self.d_peaks.cpu().numpy().view(pdfft.PEAK_DTYPE).reshape(self.n_rot, K), self.d_nout.cpu().numpy()
+
+    def close(self):
+        self.scan.close()
+
+
+def assign_lanes(pairs, n_lanes):
+    """Longest-processing-time-first assignment; pairs of a lane in that order."""
+    order = sorted(range(len(pairs)), key=lambda i: -pairs[i].cost)
+    load = [0] * n_lanes
+    lane = [0] * len(pairs)
+    for i in order:
+        l = int(np.argmin(load))
+        lane[i] = l
+        load[l] += pairs[i].cost
+    return order, lane
+
+
+def make_graph(pairs, n_lanes):
+    order, lane = assign_lanes(pairs, n_lanes)
+    spec = [(pairs[i].scan, pairs[i].n_rot, pairs[i].d_rots.data_ptr(), pairs[i].d_peaks.data_ptr(),
+             pairs[i].d_nout.data_ptr(), lane[i]) for i in order]
+    return pdfft.Graph(spec, n_lanes)
+
+
+def autotune(pairs, torch, batches=(32, 64, 128), lanes=(1, 2, 4), sample=640, log=print):
+    """Per edge: rotations per launch with the lowest time per rotation on a
+    sample of the first pair of that edge; then the lane count of the graph."""
+    tuned = {}
+    for e in sorted({p.spec["edge"] for p in pairs}):
+        p = next(q for q in pairs if q.spec["edge"] == e)
+        n = min(sample, p.n_rot)
+        best = None
+        for b in batches:
+            p.scan.set_batch(b)
+            p.run(n)  # warm
+            t0 = time.perf_counter()
+            p.run(n)
+            dt = time.perf_counter() - t0
+            log(f"autotune edge {e}: batch {b}: {dt / n * 1e6:.2f} us/rotation")
+            if best is None or dt < best[1]:
+                best = (b, dt)
+        tuned[e] = best[0]
+        for q in pairs:
+            if q.spec["edge"] == e:
+                q.scan.set_batch(best[0])
+    best_l = None
+    for nl in lanes:
+        g = make_graph(pairs, nl)
+        g.launch()
+        g.sync()
+        t0 = time.perf_counter()
+        g.launch()
+        g.sync()
+        dt = time.perf_counter() - t0
+        g.close()
+        log(f"autotune lanes {nl}: {dt * 1e3:.1f} ms per replay")
+        if best_l is None or dt < best_l[1]:
+            best_l = (nl, dt)
+    return tuned, best_l[0]
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description="batched uneven pairs in one HIP graph")
+    ap.add_argument("--pairs", type=int, default=32)
+    ap.add_argument("--rotations", type=int, default=5000)
+    ap.add_argument("--seed", type=int, default=3000)
+    ap.add_argument("--replays", type=int, default=3)
+    a = ap.parse_args(argv)
+    log = lambda m: print(f"[batch] {m}", file=sys.stderr, flush=True)  # noqa: E731
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("the scan runs on the GPU only")
+    dev = torch.device("cuda:0")
+    specs = uneven_pairs(a.pairs, a.rotations, a.seed)
+    t0 = time.perf_counter()
+    pairs = [Pair(s, torch, dev) for s in specs]
+    log(f"{len(pairs)} pairs set up in {time.perf_counter() - t0:.1f} s; edges "
+        f"{dict(zip(*np.unique([s['edge'] for s in specs], return_counts=True)))}")
+    total = sum(p.n_rot for p in pairs)
+    # eager reference: one pair after another (one rotation range at a time)
+    for p in pairs:
+        p.run()
+    t0 = time.perf_counter()
+    for p in pairs:
+        p.run()
+    t_eager = time.perf_counter() - t0
+    eager = [p.peaks() for p in pairs]
+    log(f"eager, pair by pair (batch 64): {t_eager * 1e3:.1f} ms, {total / t_eager:.0f} rotations/s")
+    tuned, n_lanes = autotune(pairs, torch, log=log)
+    for p in pairs:
+        p.d_peaks.zero_()
+        p.d_nout.zero_()
+    torch.cuda.synchronize()
+    g = make_graph(pairs, n_lanes)
+    times = []
+    same = True
+    for r in range(a.replays):
+        t0 = time.perf_counter()
+        g.launch()
+        g.sync()
+        times.append(time.perf_counter() - t0)
+        for p, (pk, no) in zip(pairs, eager):
+            gp, gn = p.peaks()
+            same &= bool(np.array_equal(gn, no) and np.array_equal(gp.view(np.uint8), pk.view(np.uint8)))
+        # eager work between replays
+        pairs[0].run(min(64, pairs[0].n_rot))
+        pairs[0].run()
+    g.close()
+    t_graph = float(np.median(times))
+    log(f"graph ({n_lanes} lanes): {t_graph * 1e3:.1f} ms, {total / t_graph:.0f} rotations/s; identical: {same}")
+    out = {
+        "workload": f"batched-uneven: {len(pairs)} pairs, receptor 800-6000 / ligand 200-2000 atoms "
+                    f"(log-uniform, seed {a.seed}), {a.rotations} rotations each, 4 terms, "
+                    f"h from {STEPS} A",
+        "pairs": len(pairs), "rotations": total,
+        "edges": {int(k): int(v) for k, v in zip(*np.unique([s["edge"] for s in specs], return_counts=True))},
+        "eager_pair_by_pair_s": t_eager, "graph_s": t_graph,
+        "rotations_per_s_eager": total / t_eager, "rotations_per_s_graph": total / t_graph,
+        "tuned_batch_per_edge": {int(k): int(v) for k, v in tuned.items()}, "lanes": n_lanes,
+        "graph_identical_to_eager": same,
+    }
+    print(json.dumps(out), flush=True)
+    for p in pairs:
+        p.close()
+    return 0 if same else 1
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -191,6 +191,18 @@ void free_batch(pd_scan* h) {
   hipFree(h->coff); h->coff = nullptr;
 }
 
+// Per-batch buffers for h->B rotations (sizes follow the ligand box).
+pd_status alloc_batch(pd_scan* h) {
+  const ScanGeom& g = h->g;
+  const size_t B = h->B;
+  PD_HIP(hipMalloc(&h->cells, sizeof(int) * B * h->n_lig));
+  PD_HIP(hipMalloc(&h->L1, sizeof(float2) * B * g.T * (size_t)g.p[0] * g.M));
+  PD_HIP(hipMalloc(&h->S1, sizeof(float2) * B * (size_t)g.n0 * g.M));
+  PD_HIP(hipMalloc(&h->cval, sizeof(float) * B * g.n0 * h->K));
+  PD_HIP(hipMalloc(&h->coff, sizeof(int) * B * g.n0 * h->K));
+  return PD_OK;
+}
+
 pd_status make_twiddles(pd_scan* h, int n, float2** out) {
   PD_HIP(hipMalloc(out, sizeof(float2) * (size_t)n));
   hipLaunchKernelGGL(k_twiddles, dim3((n + 255) / 256), dim3(256), 0, h->stream, n, *out);
@@ -621,13 +633,27 @@ pd_status pd_scan_set_ligand(pd_scan* h, int n_atoms, const double* pos, const d
   PD_HIP(hipMemcpy(h->w64, weights, sizeof(double) * nw, hipMemcpyHostToDevice));
   PD_HIP(hipMemcpy(h->w32, w32.data(), sizeof(float) * nw, hipMemcpyHostToDevice));
   h->n_lig = n_atoms;
-  const size_t B = h->B;
-  PD_HIP(hipMalloc(&h->cells, sizeof(int) * B * n_atoms));
-  PD_HIP(hipMalloc(&h->L1, sizeof(float2) * B * g.T * (size_t)g.p[0] * g.M));
-  PD_HIP(hipMalloc(&h->S1, sizeof(float2) * B * (size_t)g.n0 * g.M));
-  PD_HIP(hipMalloc(&h->cval, sizeof(float) * B * g.n0 * h->K));
-  PD_HIP(hipMalloc(&h->coff, sizeof(int) * B * g.n0 * h->K));
+  pd_status st = alloc_batch(h);
+  if (st != PD_OK) return st;
   h->ligand_ready = true;
+  return PD_OK;
+}
+
+pd_status pd_scan_set_batch(pd_scan* h, int batch) {
+  if (!h) return fail(PD_ERR_INVALID_ARGUMENT, "null handle");
+  if (batch < 1 || batch > 4096) return fail(PD_ERR_INVALID_ARGUMENT, "batch must be 1..4096");
+  PD_HIP(hipSetDevice(h->d.device));
+  PD_HIP(hipStreamSynchronize(h->stream));
+  if (batch == h->B) return PD_OK;
+  free_batch(h);
+  h->B = batch;
+  if (h->ligand_ready) {
+    pd_status st = alloc_batch(h);
+    if (st != PD_OK) {
+      h->ligand_ready = false;
+      return st;
+    }
+  }
   return PD_OK;
 }
 
@@ -726,5 +752,122 @@ pd_status pd_scan_kernel_stats(pd_scan* h, int k, double* total_ms, long long* l
 }
 
 void* pd_scan_stream(pd_scan* h) { return h ? (void*)h->stream : nullptr; }
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Several scans in one captured graph
+// ---------------------------------------------------------------------------
+struct pd_graph {
+  int device = 0;
+  hipStream_t origin = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<pd_scan*> scans;
+  std::vector<hipEvent_t> events;
+};
+
+namespace {
+void graph_free(pd_graph* g) {
+  if (!g) return;
+  if (g->exec) hipGraphExecDestroy(g->exec);
+  if (g->graph) hipGraphDestroy(g->graph);
+  for (hipEvent_t e : g->events) hipEventDestroy(e);
+  if (g->origin) hipStreamDestroy(g->origin);
+  delete g;
+}
+}  // namespace
+
+extern "C" {
+
+pd_status pd_graph_create(int n, pd_scan* const* scans, const int* n_rot, const double* const* d_rots,
+                          pd_peak* const* d_peaks, int* const* d_n_out, const int* lane, int n_lanes,
+                          pd_graph** out) {
+  if (!out || n < 1 || !scans || !n_rot || !d_rots || !d_peaks || !d_n_out || !lane || n_lanes < 1)
+    return fail(PD_ERR_INVALID_ARGUMENT, "bad graph arguments");
+  *out = nullptr;
+  const int dev = scans[0]->d.device;
+  for (int i = 0; i < n; ++i) {
+    pd_scan* h = scans[i];
+    if (!h || h->d.device != dev) return fail(PD_ERR_INVALID_ARGUMENT, "scans must share one device");
+    if (!h->receptor_ready || !h->ligand_ready) return fail(PD_ERR_STATE, "a scan has no receptor or ligand");
+    if (h->timing) return fail(PD_ERR_STATE, "kernel timing must be off while a graph is captured");
+    if (lane[i] < 0 || lane[i] >= n_lanes) return fail(PD_ERR_INVALID_ARGUMENT, "lane out of range");
+    if (n_rot[i] < 1 || !d_rots[i] || !d_peaks[i] || !d_n_out[i]) return fail(PD_ERR_INVALID_ARGUMENT, "bad pair");
+    for (int j = 0; j < i; ++j)
+      if (scans[j] == h) return fail(PD_ERR_INVALID_ARGUMENT, "a scan appears twice");
+  }
+  PD_HIP(hipSetDevice(dev));
+  pd_graph* g = new pd_graph();
+  g->device = dev;
+  g->scans.assign(scans, scans + n);
+  auto bad = [&](pd_status s) {
+    graph_free(g);
+    return s;
+  };
+  if (hipStreamCreateWithFlags(&g->origin, hipStreamNonBlocking) != hipSuccess)
+    return bad(fail(PD_ERR_DEVICE, "hipStreamCreate failed"));
+  for (int i = 0; i < n; ++i)
+    if (hipStreamSynchronize(scans[i]->stream) != hipSuccess) return bad(fail(PD_ERR_DEVICE, "sync failed"));
+  g->events.resize(n + 1);
+  for (auto& e : g->events)
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
+      return bad(fail(PD_ERR_DEVICE, "hipEventCreate failed"));
+  // Fork: every lane starts from the origin; pairs of a lane run in the given
+  // order, each on its own scan's stream; the origin joins the last of each lane.
+  if (hipStreamBeginCapture(g->origin, hipStreamCaptureModeThreadLocal) != hipSuccess)
+    return bad(fail(PD_ERR_DEVICE, "hipStreamBeginCapture failed"));
+  pd_status st = PD_OK;
+  std::vector<hipEvent_t> last(n_lanes, g->events[n]);
+  if (hipEventRecord(g->events[n], g->origin) != hipSuccess) st = fail(PD_ERR_DEVICE, "hipEventRecord failed");
+  for (int i = 0; i < n && st == PD_OK; ++i) {
+    pd_scan* h = scans[i];
+    if (hipStreamWaitEvent(h->stream, last[lane[i]], 0) != hipSuccess) {
+      st = fail(PD_ERR_DEVICE, "hipStreamWaitEvent failed");
+      break;
+    }
+    st = run_batches(h, n_rot[i], d_rots[i], 0, reinterpret_cast<PeakOut*>(d_peaks[i]), d_n_out[i]);
+    if (st != PD_OK) break;
+    if (hipEventRecord(g->events[i], h->stream) != hipSuccess) st = fail(PD_ERR_DEVICE, "hipEventRecord failed");
+    last[lane[i]] = g->events[i];
+  }
+  for (int l = 0; l < n_lanes && st == PD_OK; ++l)
+    if (last[l] != g->events[n] && hipStreamWaitEvent(g->origin, last[l], 0) != hipSuccess)
+      st = fail(PD_ERR_DEVICE, "hipStreamWaitEvent failed");
+  const hipError_t ec = hipStreamEndCapture(g->origin, &g->graph);
+  if (st != PD_OK) return bad(st);
+  if (ec != hipSuccess) return bad(fail(PD_ERR_DEVICE, std::string("capture failed: ") + hipGetErrorString(ec)));
+  if (hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0) != hipSuccess)
+    return bad(fail(PD_ERR_DEVICE, "hipGraphInstantiate failed"));
+  *out = g;
+  return PD_OK;
+}
+
+pd_status pd_graph_launch(pd_graph* g) {
+  if (!g) return fail(PD_ERR_INVALID_ARGUMENT, "null graph");
+  PD_HIP(hipSetDevice(g->device));
+  PD_HIP(hipGraphLaunch(g->exec, g->origin));
+  return PD_OK;
+}
+
+pd_status pd_graph_sync(pd_graph* g) {
+  if (!g) return fail(PD_ERR_INVALID_ARGUMENT, "null graph");
+  PD_HIP(hipSetDevice(g->device));
+  PD_HIP(hipStreamSynchronize(g->origin));
+  for (pd_scan* h : g->scans) {
+    pd_status st = check_err(h);
+    if (st != PD_OK) return st;
+  }
+  return PD_OK;
+}
+
+void* pd_graph_stream(pd_graph* g) { return g ? (void*)g->origin : nullptr; }
+
+void pd_graph_destroy(pd_graph* g) {
+  if (!g) return;
+  hipSetDevice(g->device);
+  if (g->origin) hipStreamSynchronize(g->origin);
+  graph_free(g);
+}
 
 }  // extern "C"

@@ -34,7 +34,8 @@ EXPORTS = (
     "pd_scan_get_receptor_grids", "pd_scan_set_ligand", "pd_scan_run", "pd_scan_run_device",
     "pd_scan_sync", "pd_scan_rescore64", "pd_scan_correlation_grid", "pd_scan_set_timing",
     "pd_scan_kernel_stats", "pd_scan_stream", "pd_scan_destroy", "pd_last_error",
-    "pd_find_best_size", "pd_ljcomp_params",
+    "pd_find_best_size", "pd_ljcomp_params", "pd_scan_set_batch", "pd_graph_create", "pd_graph_launch",
+    "pd_graph_sync", "pd_graph_stream", "pd_graph_destroy",
 )
 
 KERNEL_NAMES = ("k_bin", "k_lig_plane_fwd", "k_col", "k_plane_inv_topk", "k_merge")
@@ -120,8 +121,18 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     lib.pd_find_best_size.argtypes = [i]
     lib.pd_find_best_size.restype = i
     lib.pd_ljcomp_params.argtypes = [i, dp, dp, dp, i, dp, dp, dp, dp, dp, dp]
+    lib.pd_scan_set_batch.argtypes = [P, i]
+    PP = C.POINTER(P)
+    lib.pd_graph_create.argtypes = [i, PP, ip, PP, PP, PP, ip, i, PP]
+    lib.pd_graph_launch.argtypes = [P]
+    lib.pd_graph_sync.argtypes = [P]
+    lib.pd_graph_stream.argtypes = [P]
+    lib.pd_graph_stream.restype = P
+    lib.pd_graph_destroy.argtypes = [P]
+    lib.pd_graph_destroy.restype = None
     for name in EXPORTS:
-        if name not in ("pd_scan_destroy", "pd_last_error", "pd_find_best_size", "pd_scan_stream"):
+        if name not in ("pd_scan_destroy", "pd_last_error", "pd_find_best_size", "pd_scan_stream",
+                        "pd_graph_stream", "pd_graph_destroy"):
             getattr(lib, name).restype = i
     _lib = lib
     return lib
@@ -279,3 +290,48 @@ class Scan:
 
     def stream(self) -> int:
         return _lib.pd_scan_stream(self._h) or 0
+
+    def set_batch(self, batch: int) -> None:
+        _check(_lib.pd_scan_set_batch(self._h, int(batch)))
+
+    @property
+    def handle(self) -> int:
+        return self._h.value
+
+
+class Graph:
+    """Several scans captured into one HIP graph (pd_graph_*).
+
+    pairs: list of (Scan, n_rot, d_rots, d_peaks, d_n_out, lane) with device
+    pointers as ints; the buffers must stay alive as long as the graph."""
+
+    def __init__(self, pairs, n_lanes: int):
+        lib = load()
+        n = len(pairs)
+        P = C.c_void_p
+        scans = (P * n)(*[p[0].handle for p in pairs])
+        nrot = (C.c_int * n)(*[int(p[1]) for p in pairs])
+        rots = (P * n)(*[int(p[2]) for p in pairs])
+        peaks = (P * n)(*[int(p[3]) for p in pairs])
+        nout = (P * n)(*[int(p[4]) for p in pairs])
+        lane = (C.c_int * n)(*[int(p[5]) for p in pairs])
+        g = P()
+        _check(lib.pd_graph_create(n, scans, nrot, rots, peaks, nout, lane, int(n_lanes), C.byref(g)))
+        self._g = g
+
+    def launch(self) -> None:
+        _check(_lib.pd_graph_launch(self._g))
+
+    def sync(self) -> None:
+        _check(_lib.pd_graph_sync(self._g))
+
+    def close(self) -> None:
+        if getattr(self, "_g", None) is not None and self._g.value:
+            _lib.pd_graph_destroy(self._g)
+            self._g = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

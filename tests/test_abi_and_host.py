@@ -9,8 +9,8 @@ from proddl2_amd import pdfft, synth
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def declared_symbols():
-    text = open(os.path.join(ROOT, "include", "pdfft.h")).read()
+def declared_symbols(header="pdfft.h"):
+    text = open(os.path.join(ROOT, "include", header)).read()
     return sorted(set(re.findall(r"\b(pd_[a-z0-9_]+)\s*\(", text)))
 
 
@@ -21,6 +21,16 @@ def test_library_exports_every_declared_symbol():
     for name in names:
         assert hasattr(lib, name), name
     assert set(names) == set(pdfft.EXPORTS)
+
+
+def test_library_exports_every_proposal_symbol():
+    from proddl2_amd import proposal
+    lib = pdfft.load()
+    names = declared_symbols("pdprop.h")
+    for name in names:
+        assert hasattr(lib, name), name
+    assert set(names) == set(proposal.EXPORTS)
+    assert lib.pd_prop_n_params() == proposal.LAYOUT["n"] == 167953
 
 
 def test_peak_struct_layout_matches_header():

@@ -1,0 +1,94 @@
+"""Several scans in one captured HIP graph (pd_graph_*) and batch-size changes
+(pd_scan_set_batch): results must be bit-identical to eager scans."""
+import numpy as np
+import pytest
+
+from proddl2_amd import pdfft, synth
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def dev():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch.device("cuda:0")
+
+
+def _pair(dev, n, n_rec, n_lig, seed, n_rot, terms=2, batch=64):
+    cx = synth.make_complex(n_rec, n_lig, seed)
+    rt, w = synth.terms_for(cx, terms)
+    sc = pdfft.Scan((n, n, n), 1.2, terms, max_n_trans=6, max_val_corr=1e16, batch=batch)
+    sc.project_receptor(cx.receptor.pos, rt)
+    sc.set_ligand(cx.ligand.pos, w)
+    rots = synth.rotation_set(n_rot, seed + 1)
+    d_r = torch.from_numpy(rots).to(dev)
+    d_p = torch.zeros((n_rot, 6 * pdfft.PEAK_DTYPE.itemsize // 4), dtype=torch.int32, device=dev)
+    d_n = torch.zeros(n_rot, dtype=torch.int32, device=dev)
+    return sc, rots, d_r, d_p, d_n
+
+
+def _peaks(d_p, d_n, n_rot):
+    torch.cuda.synchronize()
+    return d_p.cpu().numpy().copy(), d_n.cpu().numpy().copy()
+
+
+def test_batch_size_does_not_change_results(dev):
+    sc, rots, d_r, d_p, d_n = _pair(dev, 48, 700, 220, 61, 150)
+    ref, _ = sc.run(rots)
+    for b in (1, 16, 37, 150):
+        sc.set_batch(b)
+        got, _ = sc.run(rots)
+        assert np.array_equal(got.view(np.uint8), ref.view(np.uint8)), b
+    sc.close()
+
+
+def test_graph_replays_match_eager(dev):
+    pairs = [_pair(dev, 48, 700, 220, 71, 200), _pair(dev, 64, 1500, 400, 72, 130, terms=4, batch=32),
+             _pair(dev, 48, 500, 150, 73, 90, batch=16)]
+    eager = []
+    for sc, rots, d_r, d_p, d_n in pairs:
+        sc.run_device(len(rots), d_r.data_ptr(), d_p.data_ptr(), d_n.data_ptr(), 0, sync=True)
+        eager.append(_peaks(d_p, d_n, len(rots)))
+        d_p.zero_()
+        d_n.zero_()
+    torch.cuda.synchronize()
+    for lanes, lane_of in ((1, [0, 0, 0]), (2, [0, 1, 0])):
+        g = pdfft.Graph([(sc, len(rots), d_r.data_ptr(), d_p.data_ptr(), d_n.data_ptr(), lane_of[i])
+                         for i, (sc, rots, d_r, d_p, d_n) in enumerate(pairs)], lanes)
+        for replay in range(3):
+            g.launch()
+            g.sync()
+            for (sc, rots, d_r, d_p, d_n), (ep, en) in zip(pairs, eager):
+                gp, gn = _peaks(d_p, d_n, len(rots))
+                assert np.array_equal(gn, en)
+                assert np.array_equal(gp, ep), (lanes, replay)
+            # the program's own eager work between replays
+            sc0, rots0, d_r0, d_p0, d_n0 = pairs[0]
+            sc0.run_device(len(rots0), d_r0.data_ptr(), d_p0.data_ptr(), d_n0.data_ptr(), 0, sync=True)
+            for _, _, _, d_p, d_n in pairs:
+                d_p.zero_()
+                d_n.zero_()
+            torch.cuda.synchronize()
+        g.close()
+    for p in pairs:
+        p[0].close()
+
+
+def test_graph_reports_domain_error(dev):
+    sc, rots, d_r, d_p, d_n = _pair(dev, 48, 700, 220, 81, 40)
+    # move one rotation's ligand far outside the grid by scaling the matrix
+    bad = rots.copy()
+    bad[5] *= 40.0
+    d_bad = torch.from_numpy(bad).to(dev)
+    g = pdfft.Graph([(sc, len(rots), d_bad.data_ptr(), d_p.data_ptr(), d_n.data_ptr(), 0)], 1)
+    g.launch()
+    with pytest.raises(pdfft.PdError) as e:
+        g.sync()
+    assert e.value.status == 2
+    g.close()
+    # the handle is usable again afterwards
+    got, _ = sc.run(rots)
+    assert got.shape == (40, 6)
+    sc.close()
