@@ -61,7 +61,13 @@ typedef struct pd_scan_desc {
   double max_val_corr;    /* option maxValCorr (docking_app.hpp:363)                        */
   int batch;              /* rotations per kernel launch (0: 64)                             */
   int device;             /* HIP device ordinal                                              */
+  int flags;              /* PD_SCAN_* bits, 0 by default                                    */
 } pd_scan_desc;
+
+/* pd_scan_desc.flags: run the plane passes as separate row and column kernels
+ * even when the whole y-z plane fits on one CU (grids whose plane does not fit,
+ * e.g. 256^3, always use them). */
+#define PD_SCAN_SPLIT_PLANES 1
 
 typedef struct pd_term {
   int kind;               /* PD_FIELD_*                                                      */

@@ -17,6 +17,7 @@
 #include "../../include/pdfft.h"
 #include "scan_fast.hpp"
 #include "scan_kernels.hpp"
+#include "scan_split.hpp"
 
 namespace pd_detail {
 // Message of the calling thread's last failed call, shared by every source of
@@ -151,6 +152,7 @@ struct pd_scan {
   int K = 10, B = 64, C = 32;
   long long N = 0;
   const FastSet* fast = nullptr;
+  bool split = false;  // plane passes as row + column kernels (scan_split.hpp)
   HostPlan hx, hy, hz;
   FftPlan px{}, py{}, pz{};
   int4* d_stages = nullptr;  // stage tables of px, py, pz
@@ -251,6 +253,8 @@ pd_status resolve_timing(pd_scan* h) {
 }
 
 int plane_lds(const pd_scan* h) { return h->g.n1 * h->g.ls * (int)sizeof(float2); }
+int split_row_lds(const pd_scan* h) { return kSplitRows * h->g.ls * (int)sizeof(float2); }
+int split_col_lds(const pd_scan* h) { return h->g.n1 * kSplitColStride * (int)sizeof(float2); }
 int col_lds(const pd_scan* h) { return h->g.n0 * h->C * (int)sizeof(float2); }
 
 // Receptor forward transform of the FP64 grids already in h->R64.
@@ -266,8 +270,15 @@ pd_status receptor_spectra(pd_scan* h) {
   PD_HIP(hipGetLastError());
   const int ntile = (g.M + h->C - 1) / h->C;
   for (int t = 0; t < g.T; ++t) {
-    hipLaunchKernelGGL(k_rec_plane_fwd, dim3(g.n0), dim3(kPlaneThreads), plane_lds(h), h->stream, g,
-                       h->pz, h->py, h->tw2, r32 + (size_t)t * h->N, tmp);
+    if (h->split) {
+      hipLaunchKernelGGL(k_rec_rows, dim3(g.n0, (g.n1 + kSplitRows - 1) / kSplitRows), dim3(kSplitThreads),
+                         split_row_lds(h), h->stream, g, h->pz, h->tw2, r32 + (size_t)t * h->N, tmp);
+      hipLaunchKernelGGL(k_cols_fft<-1>, dim3(g.n0, (g.h2 + kSplitCols - 1) / kSplitCols), dim3(kSplitThreads),
+                         split_col_lds(h), h->stream, g, h->py, tmp, 0, g.n1);
+    } else {
+      hipLaunchKernelGGL(k_rec_plane_fwd, dim3(g.n0), dim3(kPlaneThreads), plane_lds(h), h->stream, g,
+                         h->pz, h->py, h->tw2, r32 + (size_t)t * h->N, tmp);
+    }
     PD_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_col_fwd, dim3(ntile), dim3(kColThreads), col_lds(h), h->stream, g, h->px,
                        tmp, h->Rhat + (size_t)t * g.n0 * g.M, h->C);
@@ -295,7 +306,14 @@ pd_status launch_fft_path(pd_scan* h, int nb, const double* d_rots, float* grid_
   const FastSet* f = h->fast;
   {
     Timer tm(h, 1);
-    if (f)
+    if (h->split) {
+      hipLaunchKernelGGL(k_lig_rows, dim3(g.p[0], nb, (g.p[1] + kSplitRows - 1) / kSplitRows),
+                         dim3(kSplitThreads), split_row_lds(h), h->stream, g, h->pz, h->tw2, h->cells, h->n_lig,
+                         h->w32, h->L1);
+      hipLaunchKernelGGL(k_cols_fft<-1>, dim3(nb * g.T * g.p[0], (g.h2 + kSplitCols - 1) / kSplitCols),
+                         dim3(kSplitThreads), split_col_lds(h), h->stream, g, h->py, h->L1, g.lo[1],
+                         g.lo[1] + g.p[1]);
+    } else if (f)
       hipLaunchKernelGGL(f->lig, dim3(g.p[0], nb), dim3(kPlaneThreads), plane_lds(h), h->stream, g,
                          h->tw_y, h->tw_z, h->tw2, h->cells, h->n_lig, h->w32, h->L1);
     else
@@ -319,7 +337,12 @@ pd_status launch_fft_path(pd_scan* h, int nb, const double* d_rots, float* grid_
   }
   {
     Timer tm(h, 3);
-    if (f)
+    if (h->split) {
+      hipLaunchKernelGGL(k_cols_fft<+1>, dim3(nb * g.n0, (g.h2 + kSplitCols - 1) / kSplitCols), dim3(kSplitThreads),
+                         split_col_lds(h), h->stream, g, h->py, h->S1, 0, g.n1);
+      hipLaunchKernelGGL(k_inv_rows_topk, dim3(g.n0, nb), dim3(kSplitThreads), split_row_lds(h), h->stream, g,
+                         h->pz, h->tw2, h->S1, scale, h->K, h->cval, h->coff, grid_out);
+    } else if (f)
       hipLaunchKernelGGL(f->inv, dim3(g.n0, nb), dim3(kPlaneThreads), plane_lds(h), h->stream, g, h->tw_y,
                          h->tw_z, h->tw2, h->S1, scale, h->K, h->cval, h->coff, grid_out);
     else
@@ -480,9 +503,13 @@ pd_status pd_scan_create(const pd_scan_desc* desc, pd_scan** out) {
   if ((st = make_twiddles(h, g.n2, &h->tw2)) != PD_OK) return bad(st, pd_detail::last_error());
   if (!make_plan(g.n0, h->hx) || !make_plan(g.n1, h->hy) || !make_plan(g.m2, h->hz))
     return bad(PD_ERR_UNSUPPORTED, "grid edges must factor over {2,3,5,7,11,13}");
-  if (!plan_fits(h->hy, g.h2, kPlaneThreads) || !plan_fits(h->hz, g.n1, kPlaneThreads) ||
-      !plan_fits(h->hx, h->C, kColThreads) || g.n0 * h->C > PD_COL_PER * kColThreads)
-    return bad(PD_ERR_UNSUPPORTED, "grid plane too large for the register plan");
+  h->split = (d.flags & PD_SCAN_SPLIT_PLANES) != 0 || plane_lds(h) > kMaxLds ||
+             !plan_fits(h->hy, g.h2, kPlaneThreads) || !plan_fits(h->hz, g.n1, kPlaneThreads);
+  if (!plan_fits(h->hx, h->C, kColThreads) || g.n0 * h->C > PD_COL_PER * kColThreads)
+    return bad(PD_ERR_UNSUPPORTED, "grid edge too large for the column register plan");
+  if (h->split && (!plan_fits(h->hz, kSplitRows, kSplitThreads) || !plan_fits(h->hy, kSplitCols, kSplitThreads) ||
+                   split_row_lds(h) > kMaxLds || split_col_lds(h) > kMaxLds))
+    return bad(PD_ERR_UNSUPPORTED, "grid edge too large for the split plane kernels");
   {
     std::vector<int4> all;
     all.insert(all.end(), h->hx.st.begin(), h->hx.st.end());
@@ -496,13 +523,22 @@ pd_status pd_scan_create(const pd_scan_desc* desc, pd_scan** out) {
     h->py = FftPlan{g.n1, ny, h->d_stages + nx, h->tw_y};
     h->pz = FftPlan{g.m2, nz, h->d_stages + nx + ny, h->tw_z};
   }
-  if ((st = set_lds((const void*)k_lig_plane_fwd, plane_lds(h))) != PD_OK) return bad(st, pd_detail::last_error());
-  if ((st = set_lds((const void*)k_plane_inv_topk, plane_lds(h))) != PD_OK) return bad(st, pd_detail::last_error());
-  if ((st = set_lds((const void*)k_rec_plane_fwd, plane_lds(h))) != PD_OK) return bad(st, pd_detail::last_error());
+  if (h->split) {
+    const void* rows[] = {(const void*)k_lig_rows, (const void*)k_rec_rows, (const void*)k_inv_rows_topk};
+    for (const void* fn : rows)
+      if ((st = set_lds(fn, split_row_lds(h))) != PD_OK) return bad(st, pd_detail::last_error());
+    const void* cols[] = {(const void*)k_cols_fft<-1>, (const void*)k_cols_fft<+1>};
+    for (const void* fn : cols)
+      if ((st = set_lds(fn, split_col_lds(h))) != PD_OK) return bad(st, pd_detail::last_error());
+  } else {
+    if ((st = set_lds((const void*)k_lig_plane_fwd, plane_lds(h))) != PD_OK) return bad(st, pd_detail::last_error());
+    if ((st = set_lds((const void*)k_plane_inv_topk, plane_lds(h))) != PD_OK) return bad(st, pd_detail::last_error());
+    if ((st = set_lds((const void*)k_rec_plane_fwd, plane_lds(h))) != PD_OK) return bad(st, pd_detail::last_error());
+  }
   if ((st = set_lds((const void*)k_col, col_lds(h))) != PD_OK) return bad(st, pd_detail::last_error());
   if ((st = set_lds((const void*)k_col_fwd, col_lds(h))) != PD_OK) return bad(st, pd_detail::last_error());
   if ((st = set_lds((const void*)k_merge, g.n0 * h->K * 8)) != PD_OK) return bad(st, pd_detail::last_error());
-  h->fast = find_fast(g.n0, g.n1, g.n2);
+  h->fast = h->split ? nullptr : find_fast(g.n0, g.n1, g.n2);
   if (h->fast) {
     if ((st = set_lds((const void*)h->fast->lig, plane_lds(h))) != PD_OK) return bad(st, pd_detail::last_error());
     if ((st = set_lds((const void*)h->fast->inv, plane_lds(h))) != PD_OK) return bad(st, pd_detail::last_error());
@@ -617,7 +653,7 @@ pd_status pd_scan_set_ligand(pd_scan* h, int n_atoms, const double* pos, const d
     g.lo[a] = lo;
     g.p[a] = hi - lo + 1;
   }
-  if (!plan_fits(h->hz, g.p[1], kPlaneThreads))
+  if (!h->split && !plan_fits(h->hz, g.p[1], kPlaneThreads))
     return fail(PD_ERR_UNSUPPORTED, "ligand box too large for the register plan");
   hipFree(h->q); hipFree(h->w64); hipFree(h->w32);
   h->q = nullptr; h->w64 = nullptr; h->w32 = nullptr;

@@ -1,0 +1,198 @@
+// scan_split.hpp -- plane passes split into row and column kernels, for grids
+// whose y-z half plane does not fit in LDS (e.g. 256 x 129 complex = 264 KiB
+// at 256^3) or in the register plan of the whole-plane kernels.
+//
+// Forward ligand:  k_lig_rows (deposit + r2c along z, 32 rows of the ligand box
+//                  per block) -> L1 rows; k_cols_fft<-1> (FFT along y in
+//                  place, 16 columns per block; rows outside the box are 0).
+// Inverse:         k_cols_fft<+1> on S1, then k_inv_rows_topk (c2r along z in
+//                  chunks of 32 rows, 1/N, and the plane's K best cells kept in
+//                  a block-wide list across the chunks).
+// Receptor:        k_rec_rows + k_cols_fft<-1>, then the usual k_col_fwd.
+//
+// Same arithmetic and the same selection rule ((value, offset) lexicographic,
+// NaN as +inf) as the whole-plane kernels of scan_kernels.hpp; one extra HBM
+// round trip of the plane per pass.
+#pragma once
+
+#include "scan_kernels.hpp"
+
+constexpr int kSplitRows = 32;     // rows per block in the row kernels
+constexpr int kSplitCols = 16;     // columns per block in the column kernels
+constexpr int kSplitColStride = kSplitCols + 1;
+constexpr int kSplitThreads = 256;
+
+// grid (p0, nrot, ceil(p1 / kSplitRows)), LDS kSplitRows * ls complex.
+__global__ __launch_bounds__(kSplitThreads) void k_lig_rows(ScanGeom g, FftPlan pz,
+                                                            const float2* __restrict__ tw2,
+                                                            const int* __restrict__ cells, int n_atoms,
+                                                            const float* __restrict__ w,
+                                                            float2* __restrict__ L1) {
+  extern __shared__ float2 buf[];
+  const int p = blockIdx.x, b = blockIdx.y;
+  const int r0 = g.lo[1] + blockIdx.z * kSplitRows;
+  const int nr = min(kSplitRows, g.lo[1] + g.p[1] - r0);
+  const int i0 = g.lo[0] + p;
+  const int plane = g.n1 * g.n2;
+  const int* cb = cells + (size_t)b * n_atoms;
+  float* fb = reinterpret_cast<float*>(buf);
+  const int tot = kSplitRows * g.ls;
+  for (int t = 0; t < g.T; ++t) {
+    for (int e = threadIdx.x; e < tot; e += blockDim.x) buf[e] = make_float2(0.f, 0.f);
+    __syncthreads();
+    const float* wt = w + (size_t)t * n_atoms;
+    for (int j = threadIdx.x; j < n_atoms; j += blockDim.x) {
+      const int off = cb[j];
+      if (off >= 0 && off / plane == i0) {
+        const int r = off - i0 * plane;
+        const int i1 = r / g.n2, i2 = r - i1 * g.n2;
+        if (i1 >= r0 && i1 < r0 + nr) atomicAdd(&fb[(size_t)(i1 - r0) * 2 * g.ls + i2], wt[j]);
+      }
+    }
+    __syncthreads();
+    fft_lds<-1>(buf, pz, nr, g.ls, 1);
+    r2c_post(buf, nr, g.ls, g.m2, tw2);
+    float2* dst = L1 + (((size_t)b * g.T + t) * g.p[0] + p) * (size_t)g.M + (size_t)r0 * g.h2;
+    for (int e = threadIdx.x; e < nr * g.h2; e += blockDim.x) {
+      const int rr = e / g.h2, k2 = e - rr * g.h2;
+      dst[e] = buf[rr * g.ls + k2];
+    }
+    __syncthreads();
+  }
+}
+
+// FFT along y of planes laid out [y][k2] (row stride h2), in place.  Plane q
+// starts at base + q * M; rows outside [ylo, yhi) are read as 0.
+// grid (n_planes, ceil(h2 / kSplitCols)), LDS n1 * kSplitColStride complex.
+template <int DIR>
+__global__ __launch_bounds__(kSplitThreads) void k_cols_fft(ScanGeom g, FftPlan py, float2* __restrict__ base,
+                                                            int ylo, int yhi) {
+  extern __shared__ float2 buf[];
+  const int c0 = blockIdx.y * kSplitCols;
+  const int cw = min(kSplitCols, g.h2 - c0);
+  float2* pl = base + (size_t)blockIdx.x * g.M + c0;
+  const int nel = g.n1 * kSplitCols;
+  for (int e = threadIdx.x; e < nel; e += blockDim.x) {
+    const int y = e / kSplitCols, c = e - y * kSplitCols;
+    float2 v = make_float2(0.f, 0.f);
+    if (c < cw && y >= ylo && y < yhi) v = pl[(size_t)y * g.h2 + c];
+    buf[y * kSplitColStride + c] = v;
+  }
+  __syncthreads();
+  fft_lds<DIR>(buf, py, kSplitCols, 1, kSplitColStride);
+  for (int e = threadIdx.x; e < nel; e += blockDim.x) {
+    const int y = e / kSplitCols, c = e - y * kSplitCols;
+    if (c < cw) pl[(size_t)y * g.h2 + c] = buf[y * kSplitColStride + c];
+  }
+}
+
+// Receptor rows: dense real rows -> r2c(z).  grid (n0, ceil(n1 / kSplitRows)).
+__global__ __launch_bounds__(kSplitThreads) void k_rec_rows(ScanGeom g, FftPlan pz, const float2* __restrict__ tw2,
+                                                            const float* __restrict__ R,
+                                                            float2* __restrict__ out) {
+  extern __shared__ float2 buf[];
+  const int i0 = blockIdx.x;
+  const int r0 = blockIdx.y * kSplitRows;
+  const int nr = min(kSplitRows, g.n1 - r0);
+  float* fb = reinterpret_cast<float*>(buf);
+  for (int e = threadIdx.x; e < kSplitRows * g.ls; e += blockDim.x) buf[e] = make_float2(0.f, 0.f);
+  __syncthreads();
+  const float* src = R + ((size_t)i0 * g.n1 + r0) * g.n2;
+  for (int e = threadIdx.x; e < nr * g.n2; e += blockDim.x) {
+    const int rr = e / g.n2, i2 = e - rr * g.n2;
+    fb[(size_t)rr * 2 * g.ls + i2] = src[e];
+  }
+  __syncthreads();
+  fft_lds<-1>(buf, pz, nr, g.ls, 1);
+  r2c_post(buf, nr, g.ls, g.m2, tw2);
+  float2* dst = out + (size_t)i0 * g.M + (size_t)r0 * g.h2;
+  for (int e = threadIdx.x; e < nr * g.h2; e += blockDim.x) {
+    const int rr = e / g.h2, k2 = e - rr * g.h2;
+    dst[e] = buf[rr * g.ls + k2];
+  }
+}
+
+// Inverse rows + per-plane top-K.  grid (n0, nrot), LDS kSplitRows * ls complex.
+// The plane's K best (value, offset) pairs are kept sorted in LDS while the
+// chunks of rows stream through.
+__global__ __launch_bounds__(kSplitThreads) void k_inv_rows_topk(ScanGeom g, FftPlan pz,
+                                                                 const float2* __restrict__ tw2,
+                                                                 const float2* __restrict__ S1, float scale,
+                                                                 int K, float* __restrict__ cval,
+                                                                 int* __restrict__ coff,
+                                                                 float* __restrict__ grid_out) {
+  extern __shared__ float2 buf[];
+  __shared__ float red_v[16];
+  __shared__ int red_o[16];
+  __shared__ float lv[32];
+  __shared__ int lo[32];
+  const float INF = __int_as_float(0x7f800000);
+  const int i0 = blockIdx.x, b = blockIdx.y;
+  const float2* src = S1 + ((size_t)b * g.n0 + i0) * g.M;
+  const int npl = g.n1 * g.n2;
+  const int rs = 2 * g.ls;
+  float* fb = reinterpret_cast<float*>(buf);
+  if (threadIdx.x < 32) {
+    lv[threadIdx.x] = INF;
+    lo[threadIdx.x] = INT_MAX;
+  }
+  for (int r0 = 0; r0 < g.n1; r0 += kSplitRows) {
+    const int nr = min(kSplitRows, g.n1 - r0);
+    __syncthreads();
+    for (int e = threadIdx.x; e < nr * g.h2; e += blockDim.x) {
+      const int rr = e / g.h2, k2 = e - rr * g.h2;
+      buf[rr * g.ls + k2] = src[(size_t)r0 * g.h2 + e];
+    }
+    __syncthreads();
+    c2r_pre(buf, nr, g.ls, g.m2, tw2);
+    fft_lds<+1>(buf, pz, nr, g.ls, 1);
+    float bv = INF;
+    int be = INT_MAX;
+    for (int e = threadIdx.x; e < nr * g.n2; e += blockDim.x) {
+      const int rr = e / g.n2, i2 = e - rr * g.n2;
+      const int o = r0 * g.n2 + e;
+      float v = fb[rr * rs + i2] * scale;
+      if (grid_out) grid_out[((size_t)b * g.n0 + i0) * npl + o] = v;
+      if (v != v) v = INF;
+      fb[rr * rs + i2] = v;
+      if (pd_better(v, o, bv, be)) { bv = v; be = o; }
+    }
+    __syncthreads();
+    // Move this chunk's cells that beat the list's K-th entry into the list.
+    for (;;) {
+      float v = bv;
+      int o = be;
+      block_argmin(v, o, red_v, red_o);
+      if (o == INT_MAX || !pd_better(v, o, lv[K - 1], lo[K - 1])) break;
+      if (threadIdx.x == 0) {
+        int k = K - 1;
+        while (k > 0 && pd_better(v, o, lv[k - 1], lo[k - 1])) {
+          lv[k] = lv[k - 1];
+          lo[k] = lo[k - 1];
+          --k;
+        }
+        lv[k] = v;
+        lo[k] = o;
+      }
+      if (o == be) {
+        const int rr = o / g.n2 - r0, i2 = o % g.n2;
+        fb[rr * rs + i2] = INF;
+        bv = INF;
+        be = INT_MAX;
+        for (int e = threadIdx.x; e < nr * g.n2; e += blockDim.x) {
+          const int r2 = e / g.n2, j2 = e - r2 * g.n2;
+          const float x = fb[r2 * rs + j2];
+          const int oe = r0 * g.n2 + e;
+          if (x != INF && pd_better(x, oe, bv, be)) { bv = x; be = oe; }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < K) {
+    const size_t at = ((size_t)b * g.n0 + i0) * K + threadIdx.x;
+    cval[at] = lv[threadIdx.x];
+    coff[at] = lo[threadIdx.x] == INT_MAX ? -1 : i0 * npl + lo[threadIdx.x];
+  }
+}

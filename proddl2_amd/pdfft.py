@@ -59,7 +59,11 @@ class ScanDesc(C.Structure):
         ("max_val_corr", C.c_double),
         ("batch", C.c_int),
         ("device", C.c_int),
+        ("flags", C.c_int),
     ]
+
+
+SCAN_SPLIT_PLANES = 1
 
 
 class Term(C.Structure):
@@ -174,7 +178,7 @@ class Scan:
 
     def __init__(self, n, grid_step: float, n_terms: int, max_n_trans: int = 10,
                  max_n_trans_inp: int | None = None, max_val_corr: float = 0.0,
-                 batch: int = 64, device: int = 0):
+                 batch: int = 64, device: int = 0, flags: int = 0):
         lib = load()
         d = ScanDesc()
         d.n[:] = [int(x) for x in n]
@@ -185,6 +189,7 @@ class Scan:
         d.max_val_corr = float(max_val_corr)
         d.batch = int(batch)
         d.device = int(device)
+        d.flags = int(flags)
         h = C.c_void_p()
         _check(lib.pd_scan_create(C.byref(d), C.byref(h)))
         self._h = h

@@ -41,8 +41,8 @@ def build_case(oracle, n, h, n_rec, n_lig, n_terms, seed):
     return cx, rt, w, R
 
 
-def make_scan(n, h, T, K, cx, rt, w, batch=16, max_val=1e16):
-    sc = pdfft.Scan(n, h, T, max_n_trans=K, max_val_corr=max_val, batch=batch)
+def make_scan(n, h, T, K, cx, rt, w, batch=16, max_val=1e16, flags=0):
+    sc = pdfft.Scan(n, h, T, max_n_trans=K, max_val_corr=max_val, batch=batch, flags=flags)
     sc.project_receptor(cx.receptor.pos, rt)
     sc.set_ligand(cx.ligand.pos, w)
     return sc
@@ -70,14 +70,24 @@ def test_receptor_projection_bit_identical(oracle, n, h, n_rec, n_lig, T):
     assert bad.shape[0] == 0, f"{bad.shape[0]} cells differ, first {bad[:3]}: {G[tuple(bad[0])]} vs {R[tuple(bad[0])]}"
 
 
-@pytest.mark.parametrize("n,h,n_rec,n_lig,T", CASES)
-def test_correlation_grid(oracle, n, h, n_rec, n_lig, T):
-    print(f"case correlation grid n={n} T={T}", flush=True)
+# The same cases through the split plane kernels (row and column passes),
+# which large grids such as 256^3 use.
+SPLIT = pdfft.SCAN_SPLIT_PLANES
+CASES_BOTH = [c + (0,) for c in CASES] + [
+    ((40, 36, 48), 1.5, 300, 80, 4, SPLIT),
+    ((64, 64, 64), 1.2, 1800, 450, 2, SPLIT),
+    ((96, 96, 96), 1.2, 1500, 300, 4, SPLIT),
+]
+
+
+@pytest.mark.parametrize("n,h,n_rec,n_lig,T,flags", CASES_BOTH)
+def test_correlation_grid(oracle, n, h, n_rec, n_lig, T, flags):
+    print(f"case correlation grid n={n} T={T} flags={flags}", flush=True)
     cx, rt, w, R = build_case(oracle, n, h, n_rec, n_lig, T, seed=12)
     S = oracle.receptor_spectra(R)
     rots = synth.rotation_set(3, seed=3)
     tol = tolerance(R, w, n)
-    with make_scan(n, h, T, 4, cx, rt, w) as sc:
+    with make_scan(n, h, T, 4, cx, rt, w, flags=flags) as sc:
         for r in rots:
             q = oracle.rotate_points(r, cx.ligand.pos)
             L = np.stack([oracle.project_ligand(n, h, q, w[t]) for t in range(T)])
@@ -88,17 +98,24 @@ def test_correlation_grid(oracle, n, h, n_rec, n_lig, T):
             assert err <= tol
 
 
-@pytest.mark.parametrize("n,h,n_rec,n_lig,T", CASES)
-def test_scan_peaks_match_oracle(oracle, n, h, n_rec, n_lig, T):
-    print(f"case scan peaks n={n} T={T}", flush=True)
-    K = 10
-    cx, rt, w, R = build_case(oracle, n, h, n_rec, n_lig, T, seed=13)
+@pytest.mark.parametrize("n,h,n_rec,n_lig,T,flags", CASES_BOTH)
+def test_scan_peaks_match_oracle(oracle, n, h, n_rec, n_lig, T, flags):
+    print(f"case scan peaks n={n} T={T} flags={flags}", flush=True)
+    check_peaks(oracle, n, h, n_rec, n_lig, T, flags, n_rot=12, seed=13)
+
+
+def check_peaks(oracle, n, h, n_rec, n_lig, T, flags, n_rot, seed, K=10):
+    cx, rt, w, R = build_case(oracle, n, h, n_rec, n_lig, T, seed=seed)
     S = oracle.receptor_spectra(R)
-    rots = synth.rotation_set(12, seed=4)
+    rots = synth.rotation_set(n_rot, seed=4)
     tol = tolerance(R, w, n)
     nout_o, off_o, s_o, val_o = oracle.scan(n, h, S, cx.ligand.pos, w, rots, K=K + 1, max_val=1e16, n_threads=4)
-    with make_scan(n, h, T, K, cx, rt, w, batch=5) as sc:
+    with make_scan(n, h, T, K, cx, rt, w, batch=5, flags=flags) as sc:
+        if n[0] >= 256:
+            G = sc.receptor_grids()
+            assert np.array_equal(G, R)
         peaks, nout = sc.run(rots, rot_index0=100)
+        e64 = sc.rescore64(rots[peaks["rot"].reshape(-1) - 100], peaks["s"].reshape(-1, 3))
     assert (nout == K).all()
     checked = 0
     for r in range(len(rots)):
@@ -113,8 +130,18 @@ def test_scan_peaks_match_oracle(oracle, n, h, n_rec, n_lig, T):
                 assert (peaks["s"][r, i] == s_o[r, i]).all()
                 checked += 1
             assert abs(peaks["value"][r, i] - v[i]) <= tol + 2 * tol
+    # FP64 rescoring of every reported pose agrees with its FP32 value
+    assert np.abs(e64 - peaks["value"].reshape(-1)).max() <= 3 * tol
     print(f"  checked {checked} of {len(rots) * K} ranks exactly (tol={tol:.3e})", flush=True)
     assert checked >= len(rots) * K // 2
+
+
+def test_stress_256(oracle):
+    """256^3 at 1.0 A, receptor 12,000 / ligand 3,000 atoms, 4 terms (the large-grid
+    stress configuration): a 256 x 129 plane does not fit in LDS, so the scan runs
+    the split plane kernels; receptor grids bit-identical, peaks against the oracle."""
+    print("case stress 256^3", flush=True)
+    check_peaks(oracle, (256, 256, 256), 1.0, 12000, 3000, 4, 0, n_rot=2, seed=17)
 
 
 def test_rescore64_matches_direct_sum(oracle):
