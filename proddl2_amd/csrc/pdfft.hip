@@ -50,6 +50,25 @@ const FastSet kFast[] = {PD_FAST(32),  PD_FAST(48),  PD_FAST(64),  PD_FAST(96),
                          PD_FAST(120), PD_FAST(128), PD_FAST(144), PD_FAST(160)};
 #undef PD_FAST
 
+// Column kernels depend on N0 only: they serve every grid whose N0 is listed,
+// whatever its plane kernels (whole-plane, generic or split).
+struct FastCol {
+  int n;
+  int col_lines;
+  ColFn col;
+};
+#define PD_FCOL(N) \
+  FastCol { N, 256 / ColSplit<N>::B, k_col_fast<N> }
+const FastCol kFastCol[] = {PD_FCOL(32),  PD_FCOL(48),  PD_FCOL(64),  PD_FCOL(96), PD_FCOL(120),
+                            PD_FCOL(128), PD_FCOL(144), PD_FCOL(160), PD_FCOL(256)};
+#undef PD_FCOL
+
+const FastCol* find_fast_col(int n0) {
+  for (const FastCol& f : kFastCol)
+    if (f.n == n0) return &f;
+  return nullptr;
+}
+
 const FastSet* find_fast(int n0, int n1, int n2) {
   if (n0 != n1 || n1 != n2) return nullptr;
   for (const FastSet& f : kFast)
@@ -152,6 +171,7 @@ struct pd_scan {
   int K = 10, B = 64, C = 32;
   long long N = 0;
   const FastSet* fast = nullptr;
+  const FastCol* fcol = nullptr;  // column kernel specialised on N0, if any
   bool split = false;  // plane passes as row + column kernels (scan_split.hpp)
   HostPlan hx, hy, hz;
   FftPlan px{}, py{}, pz{};
@@ -322,14 +342,21 @@ pd_status launch_fft_path(pd_scan* h, int nb, const double* d_rots, float* grid_
   }
   {
     Timer tm(h, 2);
-    const int C = f ? f->col_lines : h->C;
+    // The specialised column kernel addresses a block group's ligand planes and
+    // outputs through 32-bit buffer offsets: cap the rotations per group.
+    const FastCol* fc = h->fcol;
+    const long long lig_bytes = (long long)g.T * g.p[0] * g.M * 8, out_bytes = (long long)g.n0 * g.M * 8;
+    const long long max_bpb = std::min(0xFFFFFFFFLL / lig_bytes, 0xFFFFFFFFLL / out_bytes);
+    if (max_bpb < 1) fc = nullptr;
+    const int C = fc ? fc->col_lines : h->C;
     const int ntile = (g.M + C - 1) / C;
     int nyb = std::max(1, std::min(nb, 4096 / std::max(1, ntile)));
+    if (fc) nyb = std::max(nyb, (int)((nb + max_bpb - 1) / max_bpb));
     const int bpb = (nb + nyb - 1) / nyb;
     nyb = (nb + bpb - 1) / bpb;
-    if (f) {
+    if (fc) {
       const int nblk = 8 * ((ntile + 7) / 8) * nyb;
-      hipLaunchKernelGGL(f->col, dim3(nblk), dim3(kColThreads), 0, h->stream, g, h->tw_x, h->L1, h->Rhat,
+      hipLaunchKernelGGL(fc->col, dim3(nblk), dim3(kColThreads), 0, h->stream, g, h->tw_x, h->L1, h->Rhat,
                          h->S1, bpb, nb, nyb);
     } else
       hipLaunchKernelGGL(k_col, dim3(ntile, nyb), dim3(kColThreads), col_lds(h), h->stream, g, h->px,
@@ -539,6 +566,7 @@ pd_status pd_scan_create(const pd_scan_desc* desc, pd_scan** out) {
   if ((st = set_lds((const void*)k_col_fwd, col_lds(h))) != PD_OK) return bad(st, pd_detail::last_error());
   if ((st = set_lds((const void*)k_merge, g.n0 * h->K * 8)) != PD_OK) return bad(st, pd_detail::last_error());
   h->fast = h->split ? nullptr : find_fast(g.n0, g.n1, g.n2);
+  h->fcol = find_fast_col(g.n0);
   if (h->fast) {
     if ((st = set_lds((const void*)h->fast->lig, plane_lds(h))) != PD_OK) return bad(st, pd_detail::last_error());
     if ((st = set_lds((const void*)h->fast->inv, plane_lds(h))) != PD_OK) return bad(st, pd_detail::last_error());
