@@ -36,7 +36,7 @@ using ColFn = void (*)(ScanGeom, const float2*, const float2*, const float2*, fl
 using LigFn = void (*)(ScanGeom, const float2*, const float2*, const float2*, const int*, int,
                        const float*, float2*);
 using InvFn = void (*)(ScanGeom, const float2*, const float2*, const float2*, const float2*, float, int,
-                       float*, int*, float*);
+                       float*, int*, float*, unsigned*);
 struct FastSet {
   int n;
   int col_lines;  // columns per block of the column kernel
@@ -190,6 +190,7 @@ struct pd_scan {
   float2* S1 = nullptr;
   float* cval = nullptr;
   int* coff = nullptr;
+  unsigned* thr = nullptr;  // per-rotation selection threshold of a batch
   int* err = nullptr;
   double* d_rots = nullptr;
   size_t rots_cap = 0;
@@ -211,6 +212,7 @@ void free_batch(pd_scan* h) {
   hipFree(h->S1); h->S1 = nullptr;
   hipFree(h->cval); h->cval = nullptr;
   hipFree(h->coff); h->coff = nullptr;
+  hipFree(h->thr); h->thr = nullptr;
 }
 
 // Per-batch buffers for h->B rotations (sizes follow the ligand box).
@@ -222,6 +224,7 @@ pd_status alloc_batch(pd_scan* h) {
   PD_HIP(hipMalloc(&h->S1, sizeof(float2) * B * (size_t)g.n0 * g.M));
   PD_HIP(hipMalloc(&h->cval, sizeof(float) * B * g.n0 * h->K));
   PD_HIP(hipMalloc(&h->coff, sizeof(int) * B * g.n0 * h->K));
+  PD_HIP(hipMalloc(&h->thr, sizeof(unsigned) * B));
   return PD_OK;
 }
 
@@ -321,7 +324,7 @@ pd_status launch_fft_path(pd_scan* h, int nb, const double* d_rots, float* grid_
     Timer tm(h, 0);
     const long long tot = (long long)nb * h->n_lig;
     hipLaunchKernelGGL(k_bin, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, h->stream, g, d_rots, nb,
-                       h->q, h->n_lig, h->cells, h->err);
+                       h->q, h->n_lig, h->cells, h->err, h->thr);
   }
   const FastSet* f = h->fast;
   {
@@ -370,8 +373,8 @@ pd_status launch_fft_path(pd_scan* h, int nb, const double* d_rots, float* grid_
       hipLaunchKernelGGL(k_inv_rows_topk, dim3(g.n0, nb), dim3(kSplitThreads), split_row_lds(h), h->stream, g,
                          h->pz, h->tw2, h->S1, scale, h->K, h->cval, h->coff, grid_out);
     } else if (f)
-      hipLaunchKernelGGL(f->inv, dim3(g.n0, nb), dim3(kPlaneThreads), plane_lds(h), h->stream, g, h->tw_y,
-                         h->tw_z, h->tw2, h->S1, scale, h->K, h->cval, h->coff, grid_out);
+      hipLaunchKernelGGL(f->inv, dim3(nb, g.n0), dim3(kPlaneThreads), plane_lds(h), h->stream, g, h->tw_y,
+                         h->tw_z, h->tw2, h->S1, scale, h->K, h->cval, h->coff, grid_out, h->thr);
     else
       hipLaunchKernelGGL(k_plane_inv_topk, dim3(g.n0, nb), dim3(kPlaneThreads), plane_lds(h), h->stream,
                          g, h->pz, h->py, h->tw2, h->S1, scale, h->K, h->cval, h->coff, grid_out);

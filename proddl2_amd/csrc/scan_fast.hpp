@@ -483,7 +483,7 @@ __global__ __launch_bounds__(512, PD_PLANE_WAVES) void k_lig_plane_fwd_fast(Scan
 }
 
 // ---------------------------------------------------------------------------
-// Inverse plane pass + per-plane top-K.  grid (n0, nrot), block 512.
+// Inverse plane pass + per-plane top-K.  grid (nrot, n0), block 512.
 // ---------------------------------------------------------------------------
 template <int N1, int N2>
 __global__ __launch_bounds__(512, PD_PLANE_WAVES) void k_plane_inv_topk_fast(ScanGeom g, const float2* __restrict__ twN1,
@@ -492,7 +492,8 @@ __global__ __launch_bounds__(512, PD_PLANE_WAVES) void k_plane_inv_topk_fast(Sca
                                                              const float2* __restrict__ S1, float scale,
                                                              int K, float* __restrict__ cval,
                                                              int* __restrict__ coff,
-                                                             float* __restrict__ grid_out) {
+                                                             float* __restrict__ grid_out,
+                                                             unsigned* __restrict__ thr) {
   constexpr int M2 = N2 / 2, H2 = M2 + 1, LS = H2 | 1, NT = 512;
   using SY = Split<N1>;
   using SZ = Split<M2>;
@@ -502,8 +503,16 @@ __global__ __launch_bounds__(512, PD_PLANE_WAVES) void k_plane_inv_topk_fast(Sca
   constexpr int LZ = NT / B2, RZ = (N1 + LZ - 1) / LZ;
   extern __shared__ float2 buf[];
   const int tid = threadIdx.x;
-  const int i0 = blockIdx.x, b = blockIdx.y;
+  // grid (nrot, n0): the rotation varies fastest, so each rotation's planes run
+  // spread over time and later planes find a tight threshold
+  const int b = blockIdx.x, i0 = blockIdx.y;
   const float2* src = S1 + ((size_t)b * g.n0 + i0) * (size_t)(N1 * H2);
+  // The rotation's selection threshold, read once per block before the
+  // transforms (an older value is larger, hence still a valid bound).  The
+  // read is device-coherent: each XCD has its own L2, which the other XCDs'
+  // atomics do not update.
+  __shared__ uint32_t s_thr;
+  if (tid == 0) s_thr = thr ? __hip_atomic_load(thr + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0xFFFFFFFFu;
   // y pass: backward FFT along y of every column
   {
     const int gy = tid / LY, cy = tid % LY;
@@ -625,9 +634,20 @@ __global__ __launch_bounds__(512, PD_PLANE_WAVES) void k_plane_inv_topk_fast(Sca
       }
     }
   }
+  // Rotation-wide threshold thr[b] (order-preserving bits of an FP32 value):
+  // every finished plane lowers it to its own K-th best, which bounds the
+  // rotation's K-th best from above, so no cell above it can be selected.  A
+  // wave whose cells all lie above it skips the selection.
+  __shared__ uint64_t wtop[NT / 64][32];
+  const int lane = tid & 63, wave = tid >> 6;
+  const uint32_t tk = s_thr;
+  const float tval = tk == 0xFFFFFFFFu ? INF : pd_key_value((uint64_t)tk << 32);
+  if (__ballot(lmin <= tval) == 0ull) {
+    if (lane < K) wtop[wave][lane] = kNoKey;
+  } else {
   // The wave's K best values are all <= tau, the K-th smallest lane minimum;
   // only those enter the lane's sorted list of L keys.
-  const float tau = wave_kth_smallest(lmin, K);
+  const float tau = fminf(wave_kth_smallest(lmin, K), tval);
   uint64_t h[L];
 #pragma unroll
   for (int i = 0; i < L; ++i) h[i] = kNoKey;
@@ -651,13 +671,15 @@ __global__ __launch_bounds__(512, PD_PLANE_WAVES) void k_plane_inv_topk_fast(Sca
   // per-wave K best: K rounds of a DPP wave minimum over the list heads; the
   // lane that supplied the minimum pops it and, once its list is used up,
   // refills it with its next L keys.
-  __shared__ uint64_t wtop[NT / 64][32];
-  const int lane = tid & 63, wave = tid >> 6;
   uint64_t last = 0;
   int popped = 0;
   for (int rk = 0; rk < K; ++rk) {
     const uint64_t m = wave_min_u64(h[0]);
     if (lane == 0) wtop[wave][rk] = m;
+    if (m == kNoKey) {  // nothing left in the wave (uniform)
+      if (lane > rk && lane < K) wtop[wave][lane] = kNoKey;
+      break;
+    }
     if (m != kNoKey && h[0] == m) {
       last = m;
       list_pop(h);
@@ -683,6 +705,7 @@ __global__ __launch_bounds__(512, PD_PLANE_WAVES) void k_plane_inv_topk_fast(Sca
       }
     }
   }
+  }
   __syncthreads();
   // block merge by wave 0: NT/64 * K keys, at most 4 per lane
   if (wave == 0) {
@@ -702,6 +725,7 @@ __global__ __launch_bounds__(512, PD_PLANE_WAVES) void k_plane_inv_topk_fast(Sca
       if (lane == 0) {
         ov[rk] = m == kNoKey ? INF : pd_key_value(m);
         oo[rk] = m == kNoKey ? -1 : (int)(uint32_t)m;
+        if (thr && rk == K - 1 && m != kNoKey) atomicMin(thr + b, (unsigned)(m >> 32));
       }
       if (m != kNoKey && hh[0] == m) list_pop(hh);
     }

@@ -136,8 +136,10 @@ __global__ void k_to_float(const double* __restrict__ in, float* __restrict__ ou
 // Cell of every rotated ligand atom; -1 (and err bit 1) when outside the box.
 __global__ void k_bin(ScanGeom g, const double* __restrict__ rots, int nrot,
                       const double* __restrict__ q, int n_atoms, int* __restrict__ cells,
-                      int* __restrict__ err) {
+                      int* __restrict__ err, unsigned* __restrict__ thr) {
   const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  // reset the batch's per-rotation selection thresholds (no threshold yet)
+  if (thr && idx < nrot) thr[idx] = 0xFFFFFFFFu;
   if (idx >= (long long)nrot * n_atoms) return;
   const int b = (int)(idx / n_atoms), j = (int)(idx - (long long)b * n_atoms);
   double y[3];
