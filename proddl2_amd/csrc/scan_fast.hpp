@@ -16,6 +16,9 @@
 #ifndef PD_PLANE_WAVES
 #define PD_PLANE_WAVES 1
 #endif
+#ifndef PD_LIG_WAVES
+#define PD_LIG_WAVES PD_PLANE_WAVES
+#endif
 
 // N = A * B with B lanes per line; B = 0 marks an unsupported edge.
 template <int N>
@@ -323,7 +326,7 @@ __global__ __launch_bounds__(256, PD_COL_WAVES) void k_col_fast(ScanGeom g, cons
 // Ligand plane pass.  grid (p0, nrot), block 512, LDS N1 * LS complex.
 // ---------------------------------------------------------------------------
 template <int N1, int N2>
-__global__ __launch_bounds__(512, PD_PLANE_WAVES) void k_lig_plane_fwd_fast(ScanGeom g, const float2* __restrict__ twN1,
+__global__ __launch_bounds__(512, PD_LIG_WAVES) void k_lig_plane_fwd_fast(ScanGeom g, const float2* __restrict__ twN1,
                                                             const float2* __restrict__ twM2,
                                                             const float2* __restrict__ tw2,
                                                             const int* __restrict__ cells, int n_atoms,
@@ -448,6 +451,9 @@ __global__ __launch_bounds__(512, PD_PLANE_WAVES) void k_lig_plane_fwd_fast(Scan
       }
       __syncthreads();
       float2* dst = L1 + (((size_t)b * g.T + t) * g.p[0] + p) * (size_t)(N1 * H2);
+      // plane stores through a buffer descriptor: 32-bit offsets instead of
+      // one 64-bit address per stored value
+      const __amdgpu_buffer_rsrc_t rd = pd_rsrc(dst, (uint32_t)(N1 * H2 * 8));
 #pragma unroll
       for (int r = 0; r < RY; ++r) {
         const int col = r * LY + cy;
@@ -461,7 +467,8 @@ __global__ __launch_bounds__(512, PD_PLANE_WAVES) void k_lig_plane_fwd_fast(Scan
             rdft<B1, -1>(x);
             if (col != 0) {
 #pragma unroll
-              for (int k2 = 0; k2 < B1; ++k2) dst[(k1 + A1 * k2) * H2 + col] = x[k2];
+              for (int k2 = 0; k2 < B1; ++k2)
+                pd_st2(rd, (uint32_t)(k1 * H2 + col) * 8u, (uint32_t)(A1 * k2 * H2 * 8), x[k2]);
             } else {
               // packed spectrum C[k] parked in the unused slot column M2
 #pragma unroll

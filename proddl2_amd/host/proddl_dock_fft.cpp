@@ -271,9 +271,75 @@ std::vector<Mat> load_rotational_grid(const std::string& path) {
 // ---------------------------------------------------------------------------
 // Reference frame (docking_molstruct.hpp:199-281)
 // ---------------------------------------------------------------------------
+// Rigid transformation x -> m x + v (RotationTranslation, Geom/transformation.hpp:242-360).
+struct RT {
+  Mat m{1, 0, 0, 0, 1, 0, 0, 0, 1};
+  double v[3] = {0, 0, 0};
+};
+
+// a * b (transformation.hpp: operator*(RotationTranslation))
+RT compose(const RT& a, const RT& b) {
+  RT r;
+  for (int i = 0; i < 3; ++i) {
+    for (int j = 0; j < 3; ++j) {
+      double acc = 0;
+      for (int k = 0; k < 3; ++k) acc += a.m[3 * i + k] * b.m[3 * k + j];
+      r.m[3 * i + j] = acc;
+    }
+    double acc = 0;
+    for (int k = 0; k < 3; ++k) acc += a.m[3 * i + k] * b.v[k];
+    r.v[i] = acc + a.v[i];
+  }
+  return r;
+}
+
+// Rotation(m^T) * Translation(-v) (transformation.hpp: inverse())
+RT inverse(const RT& a) {
+  RT r;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) r.m[3 * i + j] = a.m[3 * j + i];
+  for (int i = 0; i < 3; ++i) {
+    double acc = 0;
+    for (int k = 0; k < 3; ++k) acc += r.m[3 * i + k] * (0 - a.v[k]);
+    r.v[i] = acc;
+  }
+  return r;
+}
+
+RT translation(double x, double y, double z) {
+  RT r;
+  r.v[0] = x;
+  r.v[1] = y;
+  r.v[2] = z;
+  return r;
+}
+
+// Eu_xyz::rotMatrToEulerAng (Geom/euler.hpp:58-102): the angle triple of a rotation.
+std::array<double, 3> matrix_to_euler(const Mat& m) {
+  const double half_pi = 4.0 * std::atan(1.0) / 2.;
+  double a0, a1, a2;
+  if (m[2] < 1.0) {
+    if (m[2] > -1.0) {
+      a0 = std::atan2(-m[5], m[8]);
+      a1 = std::asin(m[2]);
+      a2 = std::atan2(-m[1], m[0]);
+    } else {
+      a0 = -std::atan2(m[3], m[4]);
+      a1 = -half_pi;
+      a2 = 0.0;
+    }
+  } else {
+    a0 = std::atan2(m[3], m[4]);
+    a1 = half_pi;
+    a2 = 0.0;
+  }
+  return {-a2, -a0, -a1};
+}
+
 struct Frame {
   double box_lo[3], box_hi[3];
   double lig_size;
+  RT tR, tL;  // original -> docking frame of receptor and ligand (MolStruct::trFromIni)
 };
 
 Frame prepare_frame(MolForceParams& p, const Options& opt) {
@@ -322,6 +388,14 @@ Frame prepare_frame(MolForceParams& p, const Options& opt) {
   for (int j = 0; j < lig.n(); ++j)
     for (int d = 0; d < 3; ++d) lig.pos[3 * j + d] -= mid[d];
   Frame f;
+  // trFromIni (docking_molstruct.hpp:199-266): receptor moved by the box
+  // transform; ligand spun about its centre of mass, then its diameter centre
+  // moved to the origin (the shared box shift cancels for the ligand)
+  f.tR = translation(-c[0], -c[1], -c[2]);
+  RT rspin;
+  rspin.m = rs;
+  const RT spin_cm = compose(compose(translation(cm[0], cm[1], cm[2]), rspin), translation(-cm[0], -cm[1], -cm[2]));
+  f.tL = compose(translation(-mid[0], -mid[1], -mid[2]), spin_cm);
   f.lig_size = std::sqrt(std::max(0.0, best));
   const double h = opt.num("gridStep", 1.5);
   const double pad = h * 2. + f.lig_size / 2. + opt.num("cutOffFft");  // :236
@@ -433,13 +507,118 @@ void rot_scan(const Options& opt, MolForceParams& mfp) {
               out.c_str());
 }
 
+// ---------------------------------------------------------------------------
+// gather task: Foreman::run + writeResults (docking_app.hpp:174-281)
+// ---------------------------------------------------------------------------
+struct RotTranValue {
+  RT t;
+  double value;
+};
+
+// RotFftScanIO_Collector (docking_io_bin.hpp:81-116): records of the files named
+// in a list, one file after another; the collection ends at the first file
+// that yields no record.
+class ScanCollector {
+ public:
+  explicit ScanCollector(const std::string& list) : list_(list) {
+    if (!list_.good()) throw Error("Could not open list file: " + list);
+  }
+  bool next(Mat& rot, std::vector<std::array<double, 4>>& tv) {
+    if (cur_ && read(rot, tv)) return true;
+    std::string name;
+    if (list_.good() && std::getline(list_, name)) {
+      cur_.reset(new std::ifstream(name, std::ios::binary));
+      if (!cur_->good()) throw Error("Could not open input file: " + name);
+      return read(rot, tv);
+    }
+    return false;
+  }
+
+ private:
+  // RotFftScanIO_Bin::read_record (docking_io_bin.hpp:37-53)
+  bool read(Mat& rot, std::vector<std::array<double, 4>>& tv) {
+    int32_t sig = 0;
+    if (!cur_->read(reinterpret_cast<char*>(&sig), 4)) return false;
+    if (sig != 2014011401) throw Error("Bad record signature in a rotation scan file");
+    uint64_t n = 0;
+    cur_->read(reinterpret_cast<char*>(rot.data()), 72);
+    cur_->read(reinterpret_cast<char*>(&n), 8);
+    tv.resize(n);
+    if (n) cur_->read(reinterpret_cast<char*>(tv.data()), 32 * (std::streamsize)n);
+    if (!*cur_) throw Error("Truncated record in a rotation scan file");
+    return true;
+  }
+  std::ifstream list_;
+  std::unique_ptr<std::ifstream> cur_;
+};
+
+void gather(const Options& opt, MolForceParams& mfp) {
+  const int max_matches = opt.integer("maxNRigidMatches", 20000);
+  if (max_matches < 1) throw Error("maxNRigidMatches must be positive");
+  const double max_val = 1e16;  // rtvalQueue.init(maxNRigidMatches, 1e16)
+  Frame fr = prepare_frame(mfp, opt);
+  // BoundPriorityQueue (Common/queue.hpp:221-251) on the value alone
+  // (cmp_rt_val, docking_app.hpp:131-145): a max-heap of at most max_matches
+  // entries; a value enters only below the limit, and a full queue replaces its
+  // top only with a strictly smaller value; sort_heap leaves them ascending.
+  auto cmp = [](const RotTranValue& a, const RotTranValue& b) { return a.value < b.value; };
+  std::vector<RotTranValue> q;
+  q.reserve(max_matches);
+  ScanCollector coll(opt.str("fft_rot_scan_list"));
+  Mat rot;
+  std::vector<std::array<double, 4>> tv;
+  long long n_rec = 0, n_in = 0;
+  while (coll.next(rot, tv)) {
+    ++n_rec;
+    RT r;
+    r.m = rot;
+    for (const auto& x : tv) {
+      ++n_in;
+      RotTranValue e;
+      e.t = compose(translation(x[0], x[1], x[2]), r);  // tranVal.tran * doneRot
+      e.value = x[3];
+      if (!(e.value < max_val)) continue;
+      if ((int)q.size() >= max_matches) {
+        if (cmp(e, q.front())) {
+          std::pop_heap(q.begin(), q.end(), cmp);
+          q.pop_back();
+        } else {
+          continue;
+        }
+      }
+      q.push_back(e);
+      std::push_heap(q.begin(), q.end(), cmp);
+    }
+  }
+  std::sort_heap(q.begin(), q.end(), cmp);
+  // getResults: ToOriginalFrameTransformer, tR^-1 * t * tL (docking_molstruct.hpp:31-51)
+  const RT tRi = inverse(fr.tR);
+  // IORigid::writeCoords, format 'b' (IO/rigid.hpp:29-150): int32 signature,
+  // int32 n, n x {float value, float angles[3], float displacement[3]}
+  const std::string out = opt.str("fft_res");
+  FILE* f = std::fopen(out.c_str(), "wb");
+  if (!f) throw Error("Cannot open file for writing: " + out);
+  const int32_t sig = 2005052501, n = (int32_t)q.size();
+  std::fwrite(&sig, 4, 1, f);
+  std::fwrite(&n, 4, 1, f);
+  for (const RotTranValue& e : q) {
+    const RT t = compose(compose(tRi, e.t), fr.tL);
+    const auto ang = matrix_to_euler(t.m);
+    const float rec[7] = {(float)e.value, (float)ang[0], (float)ang[1], (float)ang[2],
+                          (float)t.v[0], (float)t.v[1], (float)t.v[2]};
+    std::fwrite(rec, 4, 7, f);
+  }
+  if (std::fclose(f) != 0) throw Error("Error when writing to file: " + out);
+  std::printf("gather: %lld records, %lld translations -> %d matches -> %s\n", n_rec, n_in, n, out.c_str());
+}
+
 void usage() {
   std::puts(
       "Options for FFT scan process:\n"
       "  --help                     produce help message\n"
       "  --options arg              file with options common for this run\n"
       "  --molforce-params arg      molforce parameters file\n"
-      "  --task arg                 type of task to perform (rot-scan)\n"
+      "  --task arg                 type of task to perform (rot-scan or gather)\n"
       "  --fft-rot-grid-start arg   start index in rot-grid\n"
       "  --fft-rot-grid-end arg     end index in rot-grid\n"
       "  --fft-rot-scan-res arg     output file of fft scan for one task\n"
@@ -479,13 +658,17 @@ int main(int argc, char** argv) {
       set("fft-rot-grid-end");
       set("fft-rot-scan-res");
     } else if (task == "gather") {
-      throw Error("not_supported_error: task 'gather' is not provided by this build");
+      set("fft-rot-scan-list");
+      set("fft-res");
     } else {
       throw Error("Option 'task' has invalid value: " + task);
     }
     if (!args.count("molforce-params")) throw Error("required option --molforce-params is missing");
     MolForceParams mfp = load_molforce(args["molforce-params"]);
-    rot_scan(opt, mfp);
+    if (task == "gather")
+      gather(opt, mfp);
+    else
+      rot_scan(opt, mfp);
   } catch (const std::exception& e) {
     std::fprintf(stderr, "error: %s\n", e.what());
     return 1;

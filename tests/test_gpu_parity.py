@@ -56,6 +56,8 @@ CASES = [
     ((64, 64, 64), 1.2, 1800, 450, 2),
     ((96, 96, 96), 1.2, 1500, 300, 4),
     ((120, 120, 120), 1.2, 2000, 500, 2),
+    # radix-7, -11 and -13 edges (findBestSize outputs such as 56, 44, 52; z half-length 26)
+    ((56, 44, 52), 1.5, 300, 80, 2),
 ]
 
 
