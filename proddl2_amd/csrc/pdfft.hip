@@ -185,12 +185,22 @@ struct pd_scan {
   double* w64 = nullptr;
   float* w32 = nullptr;
   bool ligand_ready = false;
-  int* cells = nullptr;
-  float2* L1 = nullptr;
-  float2* S1 = nullptr;
-  float* cval = nullptr;
-  int* coff = nullptr;
-  unsigned* thr = nullptr;  // per-rotation selection threshold of a batch
+  // Per-batch buffers, one set per lane: consecutive batches alternate between
+  // two streams so that one batch's kernels can share the CUs with the next
+  // batch's (the column pass waits on memory, the plane passes on VALU).
+  struct Lane {
+    int* cells = nullptr;
+    float2* L1 = nullptr;
+    float2* S1 = nullptr;
+    float* cval = nullptr;
+    int* coff = nullptr;
+    unsigned* thr = nullptr;  // per-rotation selection threshold of a batch
+    hipStream_t s = nullptr;
+    hipEvent_t done = nullptr;
+  };
+  static constexpr int kLanes = 2;
+  Lane lane[kLanes];
+  hipEvent_t fork = nullptr;
   int* err = nullptr;
   double* d_rots = nullptr;
   size_t rots_cap = 0;
@@ -207,24 +217,28 @@ struct pd_scan {
 namespace {
 
 void free_batch(pd_scan* h) {
-  hipFree(h->cells); h->cells = nullptr;
-  hipFree(h->L1); h->L1 = nullptr;
-  hipFree(h->S1); h->S1 = nullptr;
-  hipFree(h->cval); h->cval = nullptr;
-  hipFree(h->coff); h->coff = nullptr;
-  hipFree(h->thr); h->thr = nullptr;
+  for (auto& ln : h->lane) {
+    hipFree(ln.cells); ln.cells = nullptr;
+    hipFree(ln.L1); ln.L1 = nullptr;
+    hipFree(ln.S1); ln.S1 = nullptr;
+    hipFree(ln.cval); ln.cval = nullptr;
+    hipFree(ln.coff); ln.coff = nullptr;
+    hipFree(ln.thr); ln.thr = nullptr;
+  }
 }
 
 // Per-batch buffers for h->B rotations (sizes follow the ligand box).
 pd_status alloc_batch(pd_scan* h) {
   const ScanGeom& g = h->g;
   const size_t B = h->B;
-  PD_HIP(hipMalloc(&h->cells, sizeof(int) * B * h->n_lig));
-  PD_HIP(hipMalloc(&h->L1, sizeof(float2) * B * g.T * (size_t)g.p[0] * g.M));
-  PD_HIP(hipMalloc(&h->S1, sizeof(float2) * B * (size_t)g.n0 * g.M));
-  PD_HIP(hipMalloc(&h->cval, sizeof(float) * B * g.n0 * h->K));
-  PD_HIP(hipMalloc(&h->coff, sizeof(int) * B * g.n0 * h->K));
-  PD_HIP(hipMalloc(&h->thr, sizeof(unsigned) * B));
+  for (auto& ln : h->lane) {
+    PD_HIP(hipMalloc(&ln.cells, sizeof(int) * B * h->n_lig));
+    PD_HIP(hipMalloc(&ln.L1, sizeof(float2) * B * g.T * (size_t)g.p[0] * g.M));
+    PD_HIP(hipMalloc(&ln.S1, sizeof(float2) * B * (size_t)g.n0 * g.M));
+    PD_HIP(hipMalloc(&ln.cval, sizeof(float) * B * g.n0 * h->K));
+    PD_HIP(hipMalloc(&ln.coff, sizeof(int) * B * g.n0 * h->K));
+    PD_HIP(hipMalloc(&ln.thr, sizeof(unsigned) * B));
+  }
   return PD_OK;
 }
 
@@ -244,17 +258,18 @@ pd_status set_lds(const void* fn, int bytes) {
 struct Timer {
   pd_scan* h;
   int k;
+  hipStream_t s;
   hipEvent_t a = nullptr, b = nullptr;
-  Timer(pd_scan* h_, int k_) : h(h_), k(k_) {
+  Timer(pd_scan* h_, int k_, hipStream_t s_) : h(h_), k(k_), s(s_) {
     if (h->timing) {
       hipEventCreate(&a);
       hipEventCreate(&b);
-      hipEventRecord(a, h->stream);
+      hipEventRecord(a, s);
     }
   }
   ~Timer() {
     if (h->timing) {
-      hipEventRecord(b, h->stream);
+      hipEventRecord(b, s);
       h->pending.push_back({k, a, b});
     }
   }
@@ -317,34 +332,36 @@ pd_status receptor_spectra(pd_scan* h) {
 // The four FFT-path kernels of one batch of nb rotations (bin, ligand plane
 // pass, column pass, inverse plane pass + top-K).  grid_out (optional)
 // receives the scaled correlation grids.
-pd_status launch_fft_path(pd_scan* h, int nb, const double* d_rots, float* grid_out) {
+pd_status launch_fft_path(pd_scan* h, const pd_scan::Lane& ln, int nb, const double* d_rots,
+                          float* grid_out) {
+  hipStream_t stream = ln.s;
   const ScanGeom& g = h->g;
   const float scale = (float)(1.0 / (double)h->N);
   {
-    Timer tm(h, 0);
+    Timer tm(h, 0, stream);
     const long long tot = (long long)nb * h->n_lig;
-    hipLaunchKernelGGL(k_bin, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, h->stream, g, d_rots, nb,
-                       h->q, h->n_lig, h->cells, h->err, h->thr);
+    hipLaunchKernelGGL(k_bin, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, stream, g, d_rots, nb,
+                       h->q, h->n_lig, ln.cells, h->err, ln.thr);
   }
   const FastSet* f = h->fast;
   {
-    Timer tm(h, 1);
+    Timer tm(h, 1, stream);
     if (h->split) {
       hipLaunchKernelGGL(k_lig_rows, dim3(g.p[0], nb, (g.p[1] + kSplitRows - 1) / kSplitRows),
-                         dim3(kSplitThreads), split_row_lds(h), h->stream, g, h->pz, h->tw2, h->cells, h->n_lig,
-                         h->w32, h->L1);
+                         dim3(kSplitThreads), split_row_lds(h), stream, g, h->pz, h->tw2, ln.cells, h->n_lig,
+                         h->w32, ln.L1);
       hipLaunchKernelGGL(k_cols_fft<-1>, dim3(nb * g.T * g.p[0], (g.h2 + kSplitCols - 1) / kSplitCols),
-                         dim3(kSplitThreads), split_col_lds(h), h->stream, g, h->py, h->L1, g.lo[1],
+                         dim3(kSplitThreads), split_col_lds(h), stream, g, h->py, ln.L1, g.lo[1],
                          g.lo[1] + g.p[1]);
     } else if (f)
-      hipLaunchKernelGGL(f->lig, dim3(g.p[0], nb), dim3(kPlaneThreads), plane_lds(h), h->stream, g,
-                         h->tw_y, h->tw_z, h->tw2, h->cells, h->n_lig, h->w32, h->L1);
+      hipLaunchKernelGGL(f->lig, dim3(g.p[0], nb), dim3(kPlaneThreads), plane_lds(h), stream, g,
+                         h->tw_y, h->tw_z, h->tw2, ln.cells, h->n_lig, h->w32, ln.L1);
     else
-      hipLaunchKernelGGL(k_lig_plane_fwd, dim3(g.p[0], nb), dim3(kPlaneThreads), plane_lds(h), h->stream,
-                         g, h->pz, h->py, h->tw2, h->cells, h->n_lig, h->w32, h->L1);
+      hipLaunchKernelGGL(k_lig_plane_fwd, dim3(g.p[0], nb), dim3(kPlaneThreads), plane_lds(h), stream,
+                         g, h->pz, h->py, h->tw2, ln.cells, h->n_lig, h->w32, ln.L1);
   }
   {
-    Timer tm(h, 2);
+    Timer tm(h, 2, stream);
     // The specialised column kernel addresses a block group's ligand planes and
     // outputs through 32-bit buffer offsets: cap the rotations per group.
     const FastCol* fc = h->fcol;
@@ -359,25 +376,25 @@ pd_status launch_fft_path(pd_scan* h, int nb, const double* d_rots, float* grid_
     nyb = (nb + bpb - 1) / bpb;
     if (fc) {
       const int nblk = 8 * ((ntile + 7) / 8) * nyb;
-      hipLaunchKernelGGL(fc->col, dim3(nblk), dim3(kColThreads), 0, h->stream, g, h->tw_x, h->L1, h->Rhat,
-                         h->S1, bpb, nb, nyb);
+      hipLaunchKernelGGL(fc->col, dim3(nblk), dim3(kColThreads), 0, stream, g, h->tw_x, ln.L1, h->Rhat,
+                         ln.S1, bpb, nb, nyb);
     } else
-      hipLaunchKernelGGL(k_col, dim3(ntile, nyb), dim3(kColThreads), col_lds(h), h->stream, g, h->px,
-                         h->L1, h->Rhat, h->S1, h->C, bpb, nb);
+      hipLaunchKernelGGL(k_col, dim3(ntile, nyb), dim3(kColThreads), col_lds(h), stream, g, h->px,
+                         ln.L1, h->Rhat, ln.S1, h->C, bpb, nb);
   }
   {
-    Timer tm(h, 3);
+    Timer tm(h, 3, stream);
     if (h->split) {
       hipLaunchKernelGGL(k_cols_fft<+1>, dim3(nb * g.n0, (g.h2 + kSplitCols - 1) / kSplitCols), dim3(kSplitThreads),
-                         split_col_lds(h), h->stream, g, h->py, h->S1, 0, g.n1);
-      hipLaunchKernelGGL(k_inv_rows_topk, dim3(g.n0, nb), dim3(kSplitThreads), split_row_lds(h), h->stream, g,
-                         h->pz, h->tw2, h->S1, scale, h->K, h->cval, h->coff, grid_out);
+                         split_col_lds(h), stream, g, h->py, ln.S1, 0, g.n1);
+      hipLaunchKernelGGL(k_inv_rows_topk, dim3(g.n0, nb), dim3(kSplitThreads), split_row_lds(h), stream, g,
+                         h->pz, h->tw2, ln.S1, scale, h->K, ln.cval, ln.coff, grid_out);
     } else if (f)
-      hipLaunchKernelGGL(f->inv, dim3(nb, g.n0), dim3(kPlaneThreads), plane_lds(h), h->stream, g, h->tw_y,
-                         h->tw_z, h->tw2, h->S1, scale, h->K, h->cval, h->coff, grid_out, h->thr);
+      hipLaunchKernelGGL(f->inv, dim3(nb, g.n0), dim3(kPlaneThreads), plane_lds(h), stream, g, h->tw_y,
+                         h->tw_z, h->tw2, ln.S1, scale, h->K, ln.cval, ln.coff, grid_out, ln.thr);
     else
-      hipLaunchKernelGGL(k_plane_inv_topk, dim3(g.n0, nb), dim3(kPlaneThreads), plane_lds(h), h->stream,
-                         g, h->pz, h->py, h->tw2, h->S1, scale, h->K, h->cval, h->coff, grid_out);
+      hipLaunchKernelGGL(k_plane_inv_topk, dim3(g.n0, nb), dim3(kPlaneThreads), plane_lds(h), stream,
+                         g, h->pz, h->py, h->tw2, ln.S1, scale, h->K, ln.cval, ln.coff, grid_out);
   }
   PD_HIP(hipGetLastError());
   return PD_OK;
@@ -389,18 +406,41 @@ pd_status run_batches(pd_scan* h, int n_rot, const double* d_rots, int rot0, Pea
   if (!h->ligand_ready) return fail(PD_ERR_STATE, "ligand not set");
   if (n_rot < 0) return fail(PD_ERR_INVALID_ARGUMENT, "n_rot < 0");
   const ScanGeom& g = h->g;
-  for (int b0 = 0; b0 < n_rot; b0 += h->B) {
+  // Batches alternate between the lanes (one lane while kernels are timed, so
+  // that each recorded duration belongs to one kernel alone); the lanes fork
+  // from and join back into the handle's stream.
+  // Under graph capture the batches stay on the handle's stream (the two-lane
+  // fork/join is not used there).
+  const int nbatch = (n_rot + h->B - 1) / h->B;
+  if (nbatch == 0) return PD_OK;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  PD_HIP(hipStreamIsCapturing(h->stream, &cs));
+  const bool single = h->timing || cs != hipStreamCaptureStatusNone || nbatch == 1;
+  const int nl = single ? 1 : pd_scan::kLanes;
+  pd_scan::Lane own = h->lane[0];
+  own.s = h->stream;
+  if (!single) {
+    PD_HIP(hipEventRecord(h->fork, h->stream));
+    for (int l = 0; l < nl; ++l) PD_HIP(hipStreamWaitEvent(h->lane[l].s, h->fork, 0));
+  }
+  for (int i = 0; i < nbatch; ++i) {
+    const int b0 = i * h->B;
     const int nb = std::min(h->B, n_rot - b0);
-    pd_status st = launch_fft_path(h, nb, d_rots + 9 * (size_t)b0, nullptr);
+    const pd_scan::Lane& ln = single ? own : h->lane[i % nl];
+    pd_status st = launch_fft_path(h, ln, nb, d_rots + 9 * (size_t)b0, nullptr);
     if (st != PD_OK) return st;
     {
-      Timer tm(h, 4);
-      hipLaunchKernelGGL(k_merge, dim3(nb), dim3(256), (size_t)g.n0 * h->K * 8, h->stream, g, h->K,
-                         h->cval, h->coff, h->d.max_val_corr, rot0 + b0, d_peaks + (size_t)b0 * h->K,
-                         d_nout + b0);
+      Timer tm(h, 4, ln.s);
+      hipLaunchKernelGGL(k_merge, dim3(nb), dim3(256), (size_t)g.n0 * h->K * 8, ln.s, g, h->K, ln.cval,
+                         ln.coff, h->d.max_val_corr, rot0 + b0, d_peaks + (size_t)b0 * h->K, d_nout + b0);
     }
     PD_HIP(hipGetLastError());
   }
+  if (!single)
+    for (int l = 0; l < nl; ++l) {
+      PD_HIP(hipEventRecord(h->lane[l].done, h->lane[l].s));
+      PD_HIP(hipStreamWaitEvent(h->stream, h->lane[l].done, 0));
+    }
   return PD_OK;
 }
 
@@ -526,6 +566,12 @@ pd_status pd_scan_create(const pd_scan_desc* desc, pd_scan** out) {
   };
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
     return bad(PD_ERR_DEVICE, "hipStreamCreate failed");
+  for (auto& ln : h->lane)
+    if (hipStreamCreateWithFlags(&ln.s, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&ln.done, hipEventDisableTiming) != hipSuccess)
+      return bad(PD_ERR_DEVICE, "lane stream creation failed");
+  if (hipEventCreateWithFlags(&h->fork, hipEventDisableTiming) != hipSuccess)
+    return bad(PD_ERR_DEVICE, "hipEventCreate failed");
   pd_status st;
   if ((st = make_twiddles(h, g.n0, &h->tw_x)) != PD_OK) return bad(st, pd_detail::last_error());
   if ((st = make_twiddles(h, g.n1, &h->tw_y)) != PD_OK) return bad(st, pd_detail::last_error());
@@ -593,6 +639,11 @@ void pd_scan_destroy(pd_scan* h) {
   hipFree(h->R64); hipFree(h->Rhat);
   hipFree(h->q); hipFree(h->w64); hipFree(h->w32);
   hipFree(h->err); hipFree(h->d_rots); hipFree(h->d_peaks); hipFree(h->d_nout);
+  for (auto& ln : h->lane) {
+    if (ln.s) { hipStreamSynchronize(ln.s); hipStreamDestroy(ln.s); }
+    if (ln.done) hipEventDestroy(ln.done);
+  }
+  if (h->fork) hipEventDestroy(h->fork);
   if (h->stream) hipStreamDestroy(h->stream);
   delete h;
 }
@@ -711,6 +762,7 @@ pd_status pd_scan_set_batch(pd_scan* h, int batch) {
   if (batch < 1 || batch > 4096) return fail(PD_ERR_INVALID_ARGUMENT, "batch must be 1..4096");
   PD_HIP(hipSetDevice(h->d.device));
   PD_HIP(hipStreamSynchronize(h->stream));
+  for (auto& ln : h->lane) PD_HIP(hipStreamSynchronize(ln.s));
   if (batch == h->B) return PD_OK;
   free_batch(h);
   h->B = batch;
@@ -792,7 +844,9 @@ pd_status pd_scan_correlation_grid(pd_scan* h, const double* rot, float* out) {
   float* dgrid = nullptr;
   PD_HIP(hipMalloc(&dgrid, sizeof(float) * h->N));
   PD_HIP(hipMemcpy(h->d_rots, rot, sizeof(double) * 9, hipMemcpyHostToDevice));
-  st = launch_fft_path(h, 1, h->d_rots, dgrid);
+  pd_scan::Lane ln = h->lane[0];
+  ln.s = h->stream;
+  st = launch_fft_path(h, ln, 1, h->d_rots, dgrid);
   if (st != PD_OK) return st;
   PD_HIP(hipMemcpyAsync(out, dgrid, sizeof(float) * h->N, hipMemcpyDeviceToHost, h->stream));
   PD_HIP(hipStreamSynchronize(h->stream));
