@@ -94,14 +94,15 @@ def pruned_search(sc, cx, d_rots, ex_peaks, t_exhaustive_s, log):
     }
 
 
-def cpu_baseline(cx, rt, w, wl, rots, threads, log):
+def cpu_baseline(cx, rt, w, wl, rots, threads, log, per_thread=32):
     from oracle import pdoracle
     n, h, T, K = wl["n"], wl["h"], wl["n_terms"], wl["K"]
     t0 = time.perf_counter()
     R = np.stack([pdoracle.project_receptor(n, h, cx.receptor.pos, k, p, cut, md) for (k, cut, md, p) in rt])
     S = pdoracle.receptor_spectra(R)
     log(f"cpu baseline: receptor precompute {time.perf_counter() - t0:.1f} s")
-    sample = rots[: threads]
+    # a bounded sample: per_thread rotations on each thread (about 10-15 s at 128^3 x 4)
+    sample = rots[: threads * per_thread]
     t0 = time.perf_counter()
     pdoracle.scan(n, h, S, cx.ligand.pos, w, sample, K=K, max_val=1e16, n_threads=threads)
     dt = time.perf_counter() - t0
@@ -111,7 +112,7 @@ def cpu_baseline(cx, rt, w, wl, rots, threads, log):
         "unit": "rotations/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"{len(sample)} rotations of the same workload (one per thread), FP64 oracle "
+        "sample": f"{len(sample)} rotations of the same workload ({per_thread} per thread), FP64 oracle "
                   f"restatement of the reference worker loop with its own FFT (not FFTW), "
                   f"receptor precompute excluded",
     }
@@ -130,6 +131,9 @@ def main() -> int:
                     help="first copy a 1 GiB device buffer three times (known bytes) so that a counter "
                          "profile of this run can scale FETCH_SIZE / WRITE_SIZE")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--one-lane", action="store_true",
+                    help="run the batches one after another (no overlap of consecutive batches), "
+                         "so that a kernel trace times each kernel alone")
     ap.add_argument("--no-pruned", action="store_true",
                     help="skip the proposal-pruned search (second half of the metric)")
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -160,7 +164,8 @@ def main() -> int:
     rots = synth.rotation_set(n_rot, args.seed + 1)
     log(f"synthetic inputs in {time.perf_counter() - t0:.1f} s")
 
-    sc = pdfft.Scan(n, h, T, max_n_trans=K, max_val_corr=1e16, batch=args.batch)
+    sc = pdfft.Scan(n, h, T, max_n_trans=K, max_val_corr=1e16, batch=args.batch,
+                    flags=pdfft.SCAN_ONE_LANE if args.one_lane else 0)
     t0 = time.perf_counter()
     sc.project_receptor(cx.receptor.pos, rt)
     precompute_s = time.perf_counter() - t0

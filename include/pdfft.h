@@ -68,6 +68,9 @@ typedef struct pd_scan_desc {
  * even when the whole y-z plane fits on one CU (grids whose plane does not fit,
  * e.g. 256^3, always use them). */
 #define PD_SCAN_SPLIT_PLANES 1
+/* pd_scan_desc.flags: run all batches on the handle's stream (no overlap of
+ * consecutive batches), e.g. to profile each kernel alone. */
+#define PD_SCAN_ONE_LANE 2
 
 typedef struct pd_term {
   int kind;               /* PD_FIELD_*                                                      */

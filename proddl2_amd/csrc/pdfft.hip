@@ -415,7 +415,8 @@ pd_status run_batches(pd_scan* h, int n_rot, const double* d_rots, int rot0, Pea
   if (nbatch == 0) return PD_OK;
   hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
   PD_HIP(hipStreamIsCapturing(h->stream, &cs));
-  const bool single = h->timing || cs != hipStreamCaptureStatusNone || nbatch == 1;
+  const bool single = h->timing || cs != hipStreamCaptureStatusNone || nbatch == 1 ||
+                      (h->d.flags & PD_SCAN_ONE_LANE) != 0;
   const int nl = single ? 1 : pd_scan::kLanes;
   pd_scan::Lane own = h->lane[0];
   own.s = h->stream;

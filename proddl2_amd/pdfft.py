@@ -64,6 +64,7 @@ class ScanDesc(C.Structure):
 
 
 SCAN_SPLIT_PLANES = 1
+SCAN_ONE_LANE = 2
 
 
 class Term(C.Structure):
