@@ -63,6 +63,25 @@ const FastCol kFastCol[] = {PD_FCOL(32),  PD_FCOL(48),  PD_FCOL(64),  PD_FCOL(96
                             PD_FCOL(128), PD_FCOL(144), PD_FCOL(160), PD_FCOL(256)};
 #undef PD_FCOL
 
+// Row kernels of the split inverse specialised on N2.
+using RowsFn = void (*)(ScanGeom, const float2*, const float2*, const float2*, float, int, float*, int*, float*,
+                        unsigned*);
+struct FastRows {
+  int n2;
+  int lds;  // bytes: (512 / B2) rows of LS complex
+  RowsFn fn;
+};
+#define PD_FROWS(N) \
+  FastRows { N, (512 / Split<N / 2>::B) * ((N / 2 + 1) | 1) * 8, k_inv_rows_topk_fast<N> }
+const FastRows kFastRows[] = {PD_FROWS(48), PD_FROWS(64), PD_FROWS(96), PD_FROWS(128), PD_FROWS(256)};
+#undef PD_FROWS
+
+const FastRows* find_fast_rows(int n2) {
+  for (const FastRows& f : kFastRows)
+    if (f.n2 == n2) return &f;
+  return nullptr;
+}
+
 const FastCol* find_fast_col(int n0) {
   for (const FastCol& f : kFastCol)
     if (f.n == n0) return &f;
@@ -172,6 +191,7 @@ struct pd_scan {
   long long N = 0;
   const FastSet* fast = nullptr;
   const FastCol* fcol = nullptr;  // column kernel specialised on N0, if any
+  const FastRows* frows = nullptr;  // split inverse rows specialised on N2, if any
   bool split = false;  // plane passes as row + column kernels (scan_split.hpp)
   HostPlan hx, hy, hz;
   FftPlan px{}, py{}, pz{};
@@ -387,8 +407,12 @@ pd_status launch_fft_path(pd_scan* h, const pd_scan::Lane& ln, int nb, const dou
     if (h->split) {
       hipLaunchKernelGGL(k_cols_fft<+1>, dim3(nb * g.n0, (g.h2 + kSplitCols - 1) / kSplitCols), dim3(kSplitThreads),
                          split_col_lds(h), stream, g, h->py, ln.S1, 0, g.n1);
-      hipLaunchKernelGGL(k_inv_rows_topk, dim3(g.n0, nb), dim3(kSplitThreads), split_row_lds(h), stream, g,
-                         h->pz, h->tw2, ln.S1, scale, h->K, ln.cval, ln.coff, grid_out);
+      if (h->frows)
+        hipLaunchKernelGGL(h->frows->fn, dim3(nb, g.n0), dim3(512), h->frows->lds, stream, g, h->tw_z, h->tw2, ln.S1,
+                           scale, h->K, ln.cval, ln.coff, grid_out, ln.thr);
+      else
+        hipLaunchKernelGGL(k_inv_rows_topk, dim3(g.n0, nb), dim3(kSplitThreads), split_row_lds(h), stream, g,
+                           h->pz, h->tw2, ln.S1, scale, h->K, ln.cval, ln.coff, grid_out);
     } else if (f)
       hipLaunchKernelGGL(f->inv, dim3(nb, g.n0), dim3(kPlaneThreads), plane_lds(h), stream, g, h->tw_y,
                          h->tw_z, h->tw2, ln.S1, scale, h->K, ln.cval, ln.coff, grid_out, ln.thr);
@@ -617,6 +641,11 @@ pd_status pd_scan_create(const pd_scan_desc* desc, pd_scan** out) {
   if ((st = set_lds((const void*)k_merge, g.n0 * h->K * 8)) != PD_OK) return bad(st, pd_detail::last_error());
   h->fast = h->split ? nullptr : find_fast(g.n0, g.n1, g.n2);
   h->fcol = find_fast_col(g.n0);
+  if (h->split) {
+    h->frows = find_fast_rows(g.n2);
+    if (h->frows && (st = set_lds((const void*)h->frows->fn, h->frows->lds)) != PD_OK)
+      return bad(st, pd_detail::last_error());
+  }
   if (h->fast) {
     if ((st = set_lds((const void*)h->fast->lig, plane_lds(h))) != PD_OK) return bad(st, pd_detail::last_error());
     if ((st = set_lds((const void*)h->fast->inv, plane_lds(h))) != PD_OK) return bad(st, pd_detail::last_error());

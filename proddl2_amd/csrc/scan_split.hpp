@@ -5,9 +5,11 @@
 // Forward ligand:  k_lig_rows (deposit + r2c along z, 32 rows of the ligand box
 //                  per block) -> L1 rows; k_cols_fft<-1> (FFT along y in
 //                  place, 16 columns per block; rows outside the box are 0).
-// Inverse:         k_cols_fft<+1> on S1, then k_inv_rows_topk (c2r along z in
-//                  chunks of 32 rows, 1/N, and the plane's K best cells kept in
-//                  a block-wide list across the chunks).
+// Inverse:         k_cols_fft<+1> on S1, then k_inv_rows_topk_fast<N2> (register
+//                  FFTs, wave-level selection under the rotation's threshold) or,
+//                  for other N2, k_inv_rows_topk (c2r along z in chunks of 32 rows,
+//                  1/N, and the plane's K best cells kept in a block-wide list
+//                  across the chunks).
 // Receptor:        k_rec_rows + k_cols_fft<-1>, then the usual k_col_fwd.
 //
 // Same arithmetic and the same selection rule ((value, offset) lexicographic,
@@ -15,6 +17,7 @@
 // round trip of the plane per pass.
 #pragma once
 
+#include "scan_fast.hpp"
 #include "scan_kernels.hpp"
 
 constexpr int kSplitRows = 32;     // rows per block in the row kernels
@@ -195,4 +198,177 @@ __global__ __launch_bounds__(kSplitThreads) void k_inv_rows_topk(ScanGeom g, Fft
     cval[at] = lv[threadIdx.x];
     coff[at] = lo[threadIdx.x] == INT_MAX ? -1 : i0 * npl + lo[threadIdx.x];
   }
+}
+
+// ---------------------------------------------------------------------------
+// Inverse rows + top-K with register FFTs (N2 known at compile time).  grid
+// (nrot, n0): the rotation varies fastest, as in the whole-plane kernel, so
+// that the rotation-wide threshold thr[b] tightens while the planes run.
+// Chunks of LZ = 512 / B2 rows: c2r pre-step on loading from S1, half-length
+// backward four-step FFT (one LDS exchange), scale, then per wave the K best
+// under the threshold; a block-wide running list of the K best carries over
+// the chunks.  LDS LZ * LS complex.
+// ---------------------------------------------------------------------------
+
+template <int N2>
+__global__ __launch_bounds__(512) void k_inv_rows_topk_fast(ScanGeom g, const float2* __restrict__ twM2,
+                                                            const float2* __restrict__ tw2,
+                                                            const float2* __restrict__ S1, float scale, int K,
+                                                            float* __restrict__ cval, int* __restrict__ coff,
+                                                            float* __restrict__ grid_out,
+                                                            unsigned* __restrict__ thr) {
+  constexpr int M2 = N2 / 2, H2 = M2 + 1, LS = H2 | 1, NT = 512, NW = NT / 64, L = 4;
+  using SZ = Split<M2>;
+  constexpr int A2 = SZ::A, B2 = SZ::B, J2 = SZ::J;
+  constexpr int LZ = NT / B2;
+  extern __shared__ float2 buf[];
+  __shared__ uint64_t wtop[NW][32];
+  __shared__ uint64_t best[32];
+  __shared__ uint32_t s_thr;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int b = blockIdx.x, i0 = blockIdx.y;
+  const int N1 = g.n1;
+  const float2* src = S1 + ((size_t)b * g.n0 + i0) * (size_t)(N1 * H2);
+  const float INF = __int_as_float(0x7f800000);
+  const int plane = N1 * N2, base = i0 * plane;
+  const int gz = tid / LZ, rz = tid % LZ;
+  if (tid < 32) best[tid] = kNoKey;
+  if (tid == 0) s_thr = thr ? __hip_atomic_load(thr + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0xFFFFFFFFu;
+  __syncthreads();
+  for (int r0 = 0; r0 < N1; r0 += LZ) {
+    const int row = r0 + rz;
+    const bool rok = row < N1;
+    float2 z[A2];
+#pragma unroll
+    for (int m = 0; m < A2; ++m) {
+      const int k = gz + B2 * m;
+      float2 xk = make_float2(0.f, 0.f), xm = xk;
+      if (rok) {
+        xk = src[row * H2 + k];
+        xm = src[row * H2 + M2 - k];
+      }
+      if (k == 0) { xk.y = 0.f; xm.y = 0.f; }
+      z[m] = c2r_point(xk, xm, tw2[k]);
+    }
+    four_step_phase1<A2, B2, +1>(z, gz, twM2);
+    if (rok)
+#pragma unroll
+      for (int k1 = 0; k1 < A2; ++k1) buf[rz * LS + k1 * B2 + gz] = z[k1];
+    __syncthreads();
+    float2 u[J2][B2];
+#pragma unroll
+    for (int j = 0; j < J2; ++j) {
+      const int k1 = gz + B2 * j;
+      if (rok && k1 < A2) {
+#pragma unroll
+        for (int q = 0; q < B2; ++q) u[j][q] = buf[rz * LS + k1 * B2 + q];
+        rdft<B2, +1>(u[j]);
+      }
+    }
+    __syncthreads();
+    float lmin = INF;
+#pragma unroll
+    for (int j = 0; j < J2; ++j) {
+      const int k1 = gz + B2 * j;
+      if (rok && k1 < A2) {
+#pragma unroll
+        for (int k2 = 0; k2 < B2; ++k2) {
+          const int e = row * N2 + 2 * (k1 + A2 * k2);
+          float vx = u[j][k2].x * scale, vy = u[j][k2].y * scale;
+          if (grid_out) {
+            float* go = grid_out + ((size_t)b * g.n0 + i0) * (size_t)plane;
+            go[e] = vx;
+            go[e + 1] = vy;
+          }
+          if (vx != vx) vx = INF;
+          if (vy != vy) vy = INF;
+          u[j][k2] = make_float2(vx, vy);
+          lmin = fminf(lmin, fminf(vx, vy));
+        }
+      }
+    }
+    // threshold: the rotation's (read once) and this plane's running K-th best
+    const uint64_t bk = best[K - 1];
+    float tval = s_thr == 0xFFFFFFFFu ? INF : pd_key_value((uint64_t)s_thr << 32);
+    if (bk != kNoKey) tval = fminf(tval, pd_key_value(bk));
+    if (__ballot(lmin <= tval) == 0ull) {
+      if (lane < K) wtop[wave][lane] = kNoKey;
+    } else {
+      const float tau = fminf(wave_kth_smallest(lmin, K), tval);
+      uint64_t h[L];
+#pragma unroll
+      for (int i = 0; i < L; ++i) h[i] = kNoKey;
+#pragma unroll
+      for (int j = 0; j < J2; ++j) {
+        const int k1 = gz + B2 * j;
+        if (rok && k1 < A2) {
+#pragma unroll
+          for (int k2 = 0; k2 < B2; ++k2) {
+            const int e = base + row * N2 + 2 * (k1 + A2 * k2);
+            const float2 x = u[j][k2];
+            if (x.x <= tau) list_insert(h, pd_key(x.x, e));
+            if (x.y <= tau) list_insert(h, pd_key(x.y, e + 1));
+          }
+        }
+      }
+      uint64_t last = 0;
+      int popped = 0;
+      for (int rk = 0; rk < K; ++rk) {
+        const uint64_t m = wave_min_u64(h[0]);
+        if (lane == 0) wtop[wave][rk] = m;
+        if (m == kNoKey) {
+          if (lane > rk && lane < K) wtop[wave][lane] = kNoKey;
+          break;
+        }
+        if (h[0] == m) {
+          last = m;
+          list_pop(h);
+          ++popped;
+        }
+        if (h[0] == kNoKey && popped > 0 && popped % L == 0) {
+#pragma unroll
+          for (int j = 0; j < J2; ++j) {
+            const int k1 = gz + B2 * j;
+            if (rok && k1 < A2) {
+#pragma unroll
+              for (int k2 = 0; k2 < B2; ++k2) {
+                const int e = base + row * N2 + 2 * (k1 + A2 * k2);
+                const uint64_t kx = pd_key(u[j][k2].x, e), ky = pd_key(u[j][k2].y, e + 1);
+                if (kx > last) list_insert(h, kx);
+                if (ky > last) list_insert(h, ky);
+              }
+            }
+          }
+        }
+      }
+    }
+    __syncthreads();
+    // wave 0 merges the waves' candidates with the running list
+    if (wave == 0) {
+      uint64_t hh[5];
+#pragma unroll
+      for (int i = 0; i < 5; ++i) hh[i] = kNoKey;
+#pragma unroll
+      for (int q = 0; q < 5; ++q) {
+        const int idx = lane + 64 * q;
+        if (idx < NW * K)
+          list_insert(hh, wtop[idx / K][idx % K]);
+        else if (idx < NW * K + K)
+          list_insert(hh, best[idx - NW * K]);
+      }
+      for (int rk = 0; rk < K; ++rk) {
+        const uint64_t m = wave_min_u64(hh[0]);
+        if (lane == 0) best[rk] = m;
+        if (m != kNoKey && hh[0] == m) list_pop(hh);
+      }
+    }
+    __syncthreads();
+  }
+  if (tid < K) {
+    const uint64_t m = best[tid];
+    const size_t at = ((size_t)b * g.n0 + i0) * K + tid;
+    cval[at] = m == kNoKey ? INF : pd_key_value(m);
+    coff[at] = m == kNoKey ? -1 : (int)(uint32_t)m;
+  }
+  if (thr && tid == 0 && best[K - 1] != kNoKey) atomicMin(thr + b, (unsigned)(best[K - 1] >> 32));
 }

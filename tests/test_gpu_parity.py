@@ -79,6 +79,8 @@ CASES_BOTH = [c + (0,) for c in CASES] + [
     ((40, 36, 48), 1.5, 300, 80, 4, SPLIT),
     ((64, 64, 64), 1.2, 1800, 450, 2, SPLIT),
     ((96, 96, 96), 1.2, 1500, 300, 4, SPLIT),
+    # z half-length 26 has no register-FFT split: the runtime-plan row kernel
+    ((56, 44, 52), 1.5, 300, 80, 2, SPLIT),
 ]
 
 
