@@ -32,36 +32,49 @@ namespace {
 
 // Kernels specialised on a cubic grid edge (scan_fast.hpp); other grids run
 // the generic runtime-plan kernels of scan_kernels.hpp.
-using ColFn = void (*)(ScanGeom, const float2*, const float2*, const float2*, float2*, int, int, int);
+using ColFn = void (*)(ScanGeom, const float2*, const float2*, const float4*, float2*, int, int, int);
+using LanesFn = void (*)(ScanGeom, const float2*, float4*);
 using LigFn = void (*)(ScanGeom, const float2*, const float2*, const float2*, const int*, int,
                        const float*, float2*);
 using InvFn = void (*)(ScanGeom, const float2*, const float2*, const float2*, const float2*, float, int,
                        float*, int*, float*, unsigned*);
 struct FastSet {
   int n;
-  int col_lines;  // columns per block of the column kernel
-  ColFn col;
   LigFn lig;
   InvFn inv;
 };
 #define PD_FAST(N) \
-  FastSet { N, 256 / ColSplit<N>::B, k_col_fast<N>, k_lig_plane_fwd_fast<N, N>, k_plane_inv_topk_fast<N, N> }
+  FastSet { N, k_lig_plane_fwd_fast<N, N>, k_plane_inv_topk_fast<N, N> }
 const FastSet kFast[] = {PD_FAST(32),  PD_FAST(48),  PD_FAST(64),  PD_FAST(96),
                          PD_FAST(120), PD_FAST(128), PD_FAST(144), PD_FAST(160)};
 #undef PD_FAST
 
 // Column kernels depend on N0 only: they serve every grid whose N0 is listed,
-// whatever its plane kernels (whole-plane, generic or split).
+// whatever its plane kernels (whole-plane, generic or split).  Each comes in
+// four variants by the number MZ of x positions per lane that may be nonzero
+// (ceil(q A / 4), q = 1..4): the variant with the smallest MZ >= pp / B runs.
 struct FastCol {
   int n;
   int col_lines;
-  ColFn col;
+  int lanes;  // B: lanes per column; L1 holds the ligand planes in whole groups of B
+  int a;      // A = n / B positions per lane
+  int ni;     // receptor value pairs per lane and term (k_rhat_lanes)
+  LanesFn lanes_fn;
+  int mz[4];
+  ColFn col[4];
 };
-#define PD_FCOL(N) \
-  FastCol { N, 256 / ColSplit<N>::B, k_col_fast<N> }
+#define PD_MZ(N, Q) ((ColSplit<N>::A * (Q) + 3) / 4)
+#define PD_FCOL(N)                                                                                        \
+  FastCol {                                                                                               \
+    N, 256 / ColSplit<N>::B, ColSplit<N>::B, ColSplit<N>::A, ColLanes<N>::NI, k_rhat_lanes<N>,            \
+        {PD_MZ(N, 1), PD_MZ(N, 2), PD_MZ(N, 3), PD_MZ(N, 4)},                                             \
+        {k_col_fast<N, PD_MZ(N, 1)>, k_col_fast<N, PD_MZ(N, 2)>, k_col_fast<N, PD_MZ(N, 3)>,              \
+         k_col_fast<N, PD_MZ(N, 4)>}                                                                      \
+  }
 const FastCol kFastCol[] = {PD_FCOL(32),  PD_FCOL(48),  PD_FCOL(64),  PD_FCOL(96), PD_FCOL(120),
                             PD_FCOL(128), PD_FCOL(144), PD_FCOL(160), PD_FCOL(256)};
 #undef PD_FCOL
+#undef PD_MZ
 
 // Row kernels of the split inverse specialised on N2.
 using RowsFn = void (*)(ScanGeom, const float2*, const float2*, const float2*, float, int, float*, int*, float*,
@@ -199,6 +212,8 @@ struct pd_scan {
   float2 *tw_x = nullptr, *tw_y = nullptr, *tw_z = nullptr, *tw2 = nullptr;
   double* R64 = nullptr;
   float2* Rhat = nullptr;
+  float4* Rlanes = nullptr;  // Rhat in the fast column kernel's lane order
+  int rhat_shift = 0;  // x shift of the receptor grid behind Rhat (g.pl when computed)
   bool receptor_ready = false;
   int n_lig = 0;
   double* q = nullptr;
@@ -253,7 +268,10 @@ pd_status alloc_batch(pd_scan* h) {
   const size_t B = h->B;
   for (auto& ln : h->lane) {
     PD_HIP(hipMalloc(&ln.cells, sizeof(int) * B * h->n_lig));
-    PD_HIP(hipMalloc(&ln.L1, sizeof(float2) * B * g.T * (size_t)g.p[0] * g.M));
+    // planes of [pl, pl + pp) outside the ligand box are never written: zero them once
+    const size_t l1 = sizeof(float2) * B * g.T * (size_t)g.pp * g.M;
+    PD_HIP(hipMalloc(&ln.L1, l1));
+    PD_HIP(hipMemset(ln.L1, 0, l1));
     PD_HIP(hipMalloc(&ln.S1, sizeof(float2) * B * (size_t)g.n0 * g.M));
     PD_HIP(hipMalloc(&ln.cval, sizeof(float) * B * g.n0 * h->K));
     PD_HIP(hipMalloc(&ln.coff, sizeof(int) * B * g.n0 * h->K));
@@ -332,19 +350,27 @@ pd_status receptor_spectra(pd_scan* h) {
       hipLaunchKernelGGL(k_rec_rows, dim3(g.n0, (g.n1 + kSplitRows - 1) / kSplitRows), dim3(kSplitThreads),
                          split_row_lds(h), h->stream, g, h->pz, h->tw2, r32 + (size_t)t * h->N, tmp);
       hipLaunchKernelGGL(k_cols_fft<-1>, dim3(g.n0, (g.h2 + kSplitCols - 1) / kSplitCols), dim3(kSplitThreads),
-                         split_col_lds(h), h->stream, g, h->py, tmp, 0, g.n1);
+                         split_col_lds(h), h->stream, g, h->py, tmp, g.n0, g.n0, 0, 0, g.n1);
     } else {
       hipLaunchKernelGGL(k_rec_plane_fwd, dim3(g.n0), dim3(kPlaneThreads), plane_lds(h), h->stream, g,
                          h->pz, h->py, h->tw2, r32 + (size_t)t * h->N, tmp);
     }
     PD_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_col_fwd, dim3(ntile), dim3(kColThreads), col_lds(h), h->stream, g, h->px,
-                       tmp, h->Rhat + (size_t)t * g.n0 * g.M, h->C);
+                       tmp, h->Rhat + (size_t)t * g.n0 * g.M, h->C, g.pl);
+    PD_HIP(hipGetLastError());
+  }
+  if (h->fcol) {
+    const long long ntile = (g.M + h->fcol->col_lines - 1) / h->fcol->col_lines;
+    const long long tot = (long long)g.T * ntile * 4 * h->fcol->ni * 64;
+    hipLaunchKernelGGL(h->fcol->lanes_fn, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, h->stream, g, h->Rhat,
+                       h->Rlanes);
     PD_HIP(hipGetLastError());
   }
   PD_HIP(hipStreamSynchronize(h->stream));
   hipFree(r32);
   hipFree(tmp);
+  h->rhat_shift = g.pl;
   h->receptor_ready = true;
   return PD_OK;
 }
@@ -371,8 +397,8 @@ pd_status launch_fft_path(pd_scan* h, const pd_scan::Lane& ln, int nb, const dou
                          dim3(kSplitThreads), split_row_lds(h), stream, g, h->pz, h->tw2, ln.cells, h->n_lig,
                          h->w32, ln.L1);
       hipLaunchKernelGGL(k_cols_fft<-1>, dim3(nb * g.T * g.p[0], (g.h2 + kSplitCols - 1) / kSplitCols),
-                         dim3(kSplitThreads), split_col_lds(h), stream, g, h->py, ln.L1, g.lo[1],
-                         g.lo[1] + g.p[1]);
+                         dim3(kSplitThreads), split_col_lds(h), stream, g, h->py, ln.L1, g.p[0], g.pp,
+                         g.lo[0] - g.pl, g.lo[1], g.lo[1] + g.p[1]);
     } else if (f)
       hipLaunchKernelGGL(f->lig, dim3(g.p[0], nb), dim3(kPlaneThreads), plane_lds(h), stream, g,
                          h->tw_y, h->tw_z, h->tw2, ln.cells, h->n_lig, h->w32, ln.L1);
@@ -385,7 +411,7 @@ pd_status launch_fft_path(pd_scan* h, const pd_scan::Lane& ln, int nb, const dou
     // The specialised column kernel addresses a block group's ligand planes and
     // outputs through 32-bit buffer offsets: cap the rotations per group.
     const FastCol* fc = h->fcol;
-    const long long lig_bytes = (long long)g.T * g.p[0] * g.M * 8, out_bytes = (long long)g.n0 * g.M * 8;
+    const long long lig_bytes = (long long)g.T * g.pp * g.M * 8, out_bytes = (long long)g.n0 * g.M * 8;
     const long long max_bpb = std::min(0xFFFFFFFFLL / lig_bytes, 0xFFFFFFFFLL / out_bytes);
     if (max_bpb < 1) fc = nullptr;
     const int C = fc ? fc->col_lines : h->C;
@@ -396,7 +422,10 @@ pd_status launch_fft_path(pd_scan* h, const pd_scan::Lane& ln, int nb, const dou
     nyb = (nb + bpb - 1) / bpb;
     if (fc) {
       const int nblk = 8 * ((ntile + 7) / 8) * nyb;
-      hipLaunchKernelGGL(fc->col, dim3(nblk), dim3(kColThreads), 0, stream, g, h->tw_x, ln.L1, h->Rhat,
+      const int mz = (g.pp + fc->lanes - 1) / fc->lanes;  // positions per lane that hold ligand planes
+      int v = 0;
+      while (v < 3 && fc->mz[v] < mz) ++v;
+      hipLaunchKernelGGL(fc->col[v], dim3(nblk), dim3(kColThreads), 0, stream, g, h->tw_x, ln.L1, h->Rlanes,
                          ln.S1, bpb, nb, nyb);
     } else
       hipLaunchKernelGGL(k_col, dim3(ntile, nyb), dim3(kColThreads), col_lds(h), stream, g, h->px,
@@ -406,7 +435,7 @@ pd_status launch_fft_path(pd_scan* h, const pd_scan::Lane& ln, int nb, const dou
     Timer tm(h, 3, stream);
     if (h->split) {
       hipLaunchKernelGGL(k_cols_fft<+1>, dim3(nb * g.n0, (g.h2 + kSplitCols - 1) / kSplitCols), dim3(kSplitThreads),
-                         split_col_lds(h), stream, g, h->py, ln.S1, 0, g.n1);
+                         split_col_lds(h), stream, g, h->py, ln.S1, g.n0, g.n0, 0, 0, g.n1);
       if (h->frows)
         hipLaunchKernelGGL(h->frows->fn, dim3(nb, g.n0), dim3(512), h->frows->lds, stream, g, h->tw_z, h->tw2, ln.S1,
                            scale, h->K, ln.cval, ln.coff, grid_out, ln.thr);
@@ -583,6 +612,8 @@ pd_status pd_scan_create(const pd_scan_desc* desc, pd_scan** out) {
     g.lo[a] = 0;
     g.p[a] = d.n[a];
   }
+  g.pl = 0;
+  g.pp = g.n0;
   h->N = (long long)g.n0 * g.n1 * g.n2;
   h->C = std::max(1, std::min(64, 4096 / g.n0));
   auto bad = [&](pd_status s, const std::string& m) {
@@ -641,6 +672,11 @@ pd_status pd_scan_create(const pd_scan_desc* desc, pd_scan** out) {
   if ((st = set_lds((const void*)k_merge, g.n0 * h->K * 8)) != PD_OK) return bad(st, pd_detail::last_error());
   h->fast = h->split ? nullptr : find_fast(g.n0, g.n1, g.n2);
   h->fcol = find_fast_col(g.n0);
+  if (h->fcol) {
+    // the lane-order receptor spectra are addressed with 32-bit offsets
+    const long long ntile = (g.M + h->fcol->col_lines - 1) / h->fcol->col_lines;
+    if ((long long)g.T * ntile * 4 * h->fcol->ni * 1024 > 0xFFFFFFFFLL) h->fcol = nullptr;
+  }
   if (h->split) {
     h->frows = find_fast_rows(g.n2);
     if (h->frows && (st = set_lds((const void*)h->frows->fn, h->frows->lds)) != PD_OK)
@@ -654,6 +690,11 @@ pd_status pd_scan_create(const pd_scan_desc* desc, pd_scan** out) {
       hipMalloc(&h->Rhat, sizeof(float2) * (size_t)g.T * g.n0 * g.M) != hipSuccess ||
       hipMalloc(&h->err, sizeof(int)) != hipSuccess)
     return bad(PD_ERR_DEVICE, "device allocation failed");
+  if (h->fcol) {
+    const size_t ntile = (g.M + h->fcol->col_lines - 1) / h->fcol->col_lines;
+    if (hipMalloc(&h->Rlanes, sizeof(float4) * g.T * ntile * 4 * h->fcol->ni * 64) != hipSuccess)
+      return bad(PD_ERR_DEVICE, "device allocation failed");
+  }
   hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(1), 0, h->stream, h->err, 1, 0);
   if (hipStreamSynchronize(h->stream) != hipSuccess) return bad(PD_ERR_DEVICE, "stream sync failed");
   *out = h;
@@ -666,7 +707,7 @@ void pd_scan_destroy(pd_scan* h) {
   for (auto& p : h->pending) { hipEventDestroy(p.a); hipEventDestroy(p.b); }
   free_batch(h);
   hipFree(h->tw_x); hipFree(h->tw_y); hipFree(h->tw_z); hipFree(h->tw2); hipFree(h->d_stages);
-  hipFree(h->R64); hipFree(h->Rhat);
+  hipFree(h->R64); hipFree(h->Rhat); hipFree(h->Rlanes);
   hipFree(h->q); hipFree(h->w64); hipFree(h->w32);
   hipFree(h->err); hipFree(h->d_rots); hipFree(h->d_peaks); hipFree(h->d_nout);
   for (auto& ln : h->lane) {
@@ -765,6 +806,11 @@ pd_status pd_scan_set_ligand(pd_scan* h, int n_atoms, const double* pos, const d
     g.lo[a] = lo;
     g.p[a] = hi - lo + 1;
   }
+  // L1 planes: the x box from its first plane, rounded up to whole groups of
+  // the column kernel's lanes; the receptor spectra follow the shift.
+  const int al = h->fcol ? h->fcol->lanes : 1;
+  g.pl = g.lo[0];
+  g.pp = std::min(g.n0, (g.p[0] + al - 1) / al * al);
   if (!h->split && !plan_fits(h->hz, g.p[1], kPlaneThreads))
     return fail(PD_ERR_UNSUPPORTED, "ligand box too large for the register plan");
   hipFree(h->q); hipFree(h->w64); hipFree(h->w32);
@@ -783,6 +829,10 @@ pd_status pd_scan_set_ligand(pd_scan* h, int n_atoms, const double* pos, const d
   h->n_lig = n_atoms;
   pd_status st = alloc_batch(h);
   if (st != PD_OK) return st;
+  if (h->receptor_ready && h->rhat_shift != g.pl) {
+    h->receptor_ready = false;
+    if ((st = receptor_spectra(h)) != PD_OK) return st;
+  }
   h->ligand_ready = true;
   return PD_OK;
 }

@@ -90,6 +90,47 @@ __device__ __forceinline__ void four_step_phase1(float2 (&v)[A], int g, const fl
   }
 }
 
+// DFT of A values of which only the first MZ may be nonzero (natural order in
+// and out): with R = A / MZ and k = r + R q,
+//   X[r + R q] = sum_{m < MZ} (x[m] W_A^{m r}) W_MZ^{m q},
+// i.e. R twiddled DFTs of length MZ.  MZ not dividing A falls back to rdft<A>.
+template <int A, int MZ, int DIR>
+__device__ __forceinline__ void rdft_zt(float2 (&v)[A]) {
+  if constexpr (MZ >= A || A % MZ != 0) {
+    rdft<A, DIR>(v);
+  } else {
+    constexpr int R = A / MZ;
+    float2 y[R][MZ];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+#pragma unroll
+      for (int m = 0; m < MZ; ++m) {
+        const int e = (m * r) % A;
+        const float c = kCos[A][e], s = DIR * kSin[A][e];
+        const float2 a = v[m];
+        y[r][m] = e == 0 ? a : make_float2(a.x * c - a.y * s, a.x * s + a.y * c);
+      }
+      rdft<MZ, DIR>(y[r]);
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+      for (int q = 0; q < MZ; ++q) v[r + R * q] = y[r][q];
+  }
+}
+
+// Phase 1 with a zero tail: x[g + B m] = 0 for m >= MZ.
+template <int A, int B, int MZ, int DIR>
+__device__ __forceinline__ void four_step_phase1_zt(float2 (&v)[A], int g, const float2* __restrict__ twN) {
+  rdft_zt<A, MZ, DIR>(v);
+#pragma unroll
+  for (int k1 = 1; k1 < A; ++k1) {
+    float2 w = twN[g * k1];
+    if (DIR > 0) w.y = -w.y;
+    v[k1] = cmul(v[k1], w);
+  }
+}
+
 // Inverse-order variant used after accumulation: lane holds X[k1 + A k2] for
 // its k1 (all k2) in u[k2]; DFT_B over k2 (direction DIR) then twiddle by
 // W_N^{DIR-sign k1 g}, giving P[k1][g] in u[g].

@@ -33,10 +33,6 @@ struct Split {
 #ifndef PD_COL_B16
 #define PD_COL_B16 0
 #endif
-// Column pass: load the next term's ligand planes while transforming this one.
-#ifndef PD_COL_PREFETCH
-#define PD_COL_PREFETCH 1
-#endif
 template <int N>
 struct ColSplit {
   static constexpr bool W = PD_COL_B16 && N % 16 == 0 && N / 16 >= 4 && N / 16 <= 16;
@@ -53,6 +49,9 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t pd_rsrc(const void* p, uint32_
 }
 __device__ __forceinline__ float2 pd_ld2(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
   return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0));
+}
+__device__ __forceinline__ float4 pd_ld4(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+  return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff, (int)soff, 0));
 }
 __device__ __forceinline__ void pd_st2(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff, float2 v) {
   using I2 = decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0));
@@ -181,6 +180,49 @@ __device__ __forceinline__ float2 c2r_point(float2 xk, float2 xm, float2 w) {
 }
 
 // ---------------------------------------------------------------------------
+// Receptor spectra in the column pass's lane order.  Lane L = (lg, cl) of wave
+// w in column tile `tile` needs, per term t, R_t[k1 + A k2][col] for
+// k1 = lg + B j (j < J; k1 clamped to A - 1) and k2 < B, col = tile C + w CW + cl
+// (clamped to the tile's first column past the last one, as the column pass
+// clamps its lanes).  Those J B values, in order f = j B + k2, are stored in
+// pairs: pair i of lane L at float4 index (((t ntile + tile) 4 + w) NI + i) 64 + L,
+// NI = J B / 2, so that one load per pair reads 1 KiB contiguous per wave.
+// grid ceil(T * ntile * 4 * NI * 64 / 256), block 256.
+// ---------------------------------------------------------------------------
+template <int N0>
+struct ColLanes {
+  using S = ColSplit<N0>;
+  static constexpr int A = S::A, B = S::B, J = S::J, C = 256 / B, CW = 64 / B, NI = J * B / 2;
+  static_assert(B % 2 == 0, "pairs of receptor values per lane");
+};
+
+template <int N0>
+__global__ __launch_bounds__(256) void k_rhat_lanes(ScanGeom g, const float2* __restrict__ Rhat,
+                                                    float4* __restrict__ RL) {
+  using P = ColLanes<N0>;
+  const int M = g.M, ntile = (M + P::C - 1) / P::C;
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long long)g.T * ntile * 4 * P::NI * 64) return;
+  const int L = (int)(e % 64);
+  long long r = e / 64;
+  const int i = (int)(r % P::NI);
+  r /= P::NI;
+  const int w = (int)(r % 4);
+  r /= 4;
+  const int tile = (int)(r % ntile), t = (int)(r / ntile);
+  const int lg = L / P::CW, cl = L % P::CW;
+  const int m0 = tile * P::C;
+  const int c = m0 + w * P::CW + cl < M ? m0 + w * P::CW + cl : m0;
+  float2 v[2];
+  for (int q = 0; q < 2; ++q) {
+    const int f = 2 * i + q, j = f / P::B, k2 = f % P::B;
+    const int k1 = min(lg + P::B * j, P::A - 1);
+    v[q] = Rhat[((size_t)t * g.n0 + k1 + P::A * k2) * M + c];
+  }
+  RL[e] = make_float4(v[0].x, v[0].y, v[1].x, v[1].y);
+}
+
+// ---------------------------------------------------------------------------
 // Column pass.  1D grid of 8 * ceil(ntile/8) * nyb blocks, block 256,
 // C = 256/B lines per tile.  Blocks are dealt to the 8 XCDs round-robin, so
 // block id b runs on XCD b % 8; the mapping below gives every rotation group
@@ -188,12 +230,13 @@ __device__ __forceinline__ float2 c2r_point(float2 xk, float2 xm, float2 w) {
 // tile's receptor spectra (T * N0 * C complex) are read once into that XCD's
 // L2 and reused nyb times.
 // ---------------------------------------------------------------------------
-template <int N0>
+template <int N0, int MZ>
 __global__ __launch_bounds__(256, PD_COL_WAVES) void k_col_fast(ScanGeom g, const float2* __restrict__ twN0,
                                                   const float2* __restrict__ L1,
-                                                  const float2* __restrict__ Rhat,
+                                                  const float4* __restrict__ RL,
                                                   float2* __restrict__ S1, int bpb, int nrot, int nyb) {
   using S = ColSplit<N0>;
+  constexpr int NI = ColLanes<N0>::NI;
   constexpr int A = S::A, B = S::B, J = S::J, C = 256 / B;
   // Each wave owns CW = 64/B columns; lane (lg, cl) is lane lg of column cl.
   // The exchange of a column stays inside its wave: slot (k1, q, cl) at
@@ -217,54 +260,67 @@ __global__ __launch_bounds__(256, PD_COL_WAVES) void k_col_fast(ScanGeom g, cons
   if (m0 >= M) return;
   const bool colok = m0 + col < M;
   const int bbeg = rg * bpb, bend = min(nrot, bbeg + bpb);
-  const int p0 = g.p[0], lo0 = g.lo[0];
-  const uint32_t vcol = (uint32_t)col * 8u;
-  const uint32_t pM8 = (uint32_t)p0 * M * 8u;
-  // receptor spectra of all terms (T * n0 * M) and the batch's ligand planes
-  const __amdgpu_buffer_rsrc_t rr = pd_rsrc(Rhat, (uint32_t)T * g.n0 * M * 8u);
-  const __amdgpu_buffer_rsrc_t rl = pd_rsrc(L1 + (size_t)bbeg * T * p0 * M, (uint32_t)(bend - bbeg) * T * pM8);
+  // L1 holds pp planes (a multiple of B) per (rotation, term): x position
+  // lg + B m is a stored plane for every lane (m < pp / B) or for none, and
+  // positions m >= MZ >= pp / B are zero (not loaded).
+  const int pp = g.pp;
+  // Lanes past the last column (last tile only) redo column 0 of the tile:
+  // the same loads and arithmetic give the same bits, so their stores are
+  // harmless duplicates.
+  const uint32_t vcol = colok ? (uint32_t)col * 8u : 0u;
+  const uint32_t pM8 = (uint32_t)pp * M * 8u;
+  // receptor spectra of all terms in lane order (k_rhat_lanes) and the
+  // batch's ligand planes
+  const int ntile = (M + C - 1) / C;
+  const __amdgpu_buffer_rsrc_t rr = pd_rsrc(RL, (uint32_t)T * ntile * 4 * NI * 1024u);
+  const __amdgpu_buffer_rsrc_t rl = pd_rsrc(L1 + (size_t)bbeg * T * pp * M, (uint32_t)(bend - bbeg) * T * pM8);
   const __amdgpu_buffer_rsrc_t rs = pd_rsrc(S1 + (size_t)bbeg * g.n0 * M, (uint32_t)(bend - bbeg) * g.n0 * M * 8u);
-  // iteration it = (rotation bbeg + it / T, term it % T); planes of one
-  // iteration start at it * p0 * M complex in rl.
+  // zero-size buffer: every load through it is out of range and returns 0
+  const __amdgpu_buffer_rsrc_t rz = pd_rsrc(L1, 0u);
+  const int nit = (bend - bbeg) * T;
+  // Every memory instruction is issued unconditionally, so that the count of
+  // outstanding loads is the same on every path and each wait covers only
+  // what it consumes: a position m in [pp / B, MZ), or an iteration past the
+  // last, reads 0 through rz.  Iteration it = (rotation bbeg + it / T, term
+  // it % T); its planes start at it * pp * M complex in rl.
   auto load_planes = [&](int it, float2 (&x)[A]) {
     const uint32_t st = (uint32_t)it * pM8 + (uint32_t)m0 * 8u;
 #pragma unroll
     for (int m = 0; m < A; ++m) {
-      const int p = lg + B * m - lo0;
-      x[m] = (colok && p >= 0 && p < p0) ? pd_ld2(rl, (uint32_t)p * M * 8u + vcol, st) : make_float2(0.f, 0.f);
+      if (m < MZ) {
+        const bool in = it < nit && B * m < pp;
+        x[m] = pd_ld2(in ? rl : rz, in ? (uint32_t)(lg + B * m) * M * 8u + vcol : 0u, in ? st : 0u);
+      } else {
+        x[m] = make_float2(0.f, 0.f);
+      }
     }
   };
-  const int nit = (bend - bbeg) * T;
-  float2 v[A];
-  if (PD_COL_PREFETCH && nit > 0) load_planes(0, v);
   float2 acc[J][B];
 #pragma unroll
   for (int j = 0; j < J; ++j)
 #pragma unroll
     for (int k = 0; k < B; ++k) acc[j][k] = make_float2(0.f, 0.f);
-#pragma unroll 1
-  for (int it = 0; it < nit; ++it) {
+  // One iteration: the term's receptor values, the next iteration's ligand
+  // planes into vn, then the forward transform of v and the accumulation;
+  // after the last term the inverse transform and the store.
+  auto step = [&](int it, float2 (&v)[A], float2 (&vn)[A]) {
     const int t = it % T;
-    // receptor values of this term, issued before the transform
     float2 rv[J][B];
+    const uint32_t rbase = ((uint32_t)(t * ntile + tile) * 4u + (uint32_t)wave) * NI * 1024u;
 #pragma unroll
-    for (int j = 0; j < J; ++j) {
-      const int k1 = lg + B * j;
-      if (k1 < A && colok) {
-        const uint32_t vr = (uint32_t)k1 * M * 8u + vcol;
-#pragma unroll
-        for (int k2 = 0; k2 < B; ++k2)
-          rv[j][k2] = pd_ld2(rr, vr, ((uint32_t)(t * g.n0 + A * k2) * M + m0) * 8u);
-      }
+    for (int i = 0; i < NI; ++i) {
+      const float4 q = pd_ld4(rr, (uint32_t)lane * 16u, rbase + (uint32_t)i * 1024u);
+      rv[(2 * i) / B][(2 * i) % B] = make_float2(q.x, q.y);
+      rv[(2 * i + 1) / B][(2 * i + 1) % B] = make_float2(q.z, q.w);
     }
-#if PD_COL_PREFETCH
-    // next iteration's ligand planes
-    float2 vn[A];
-    if (it + 1 < nit) load_planes(it + 1, vn);
-#else
-    load_planes(it, v);
-#endif
-    four_step_phase1<A, B, -1>(v, lg, twl);
+    // Loads complete in issue order: the receptor values (L2 hits) go out
+    // before the prefetch (HBM), so that the accumulation below does not
+    // wait for the prefetch, and v's planes, older than both, are waited for
+    // alone.  The barriers keep the scheduler from reordering the groups.
+    __builtin_amdgcn_sched_barrier(0);
+    load_planes(it + 1, vn);
+    __builtin_amdgcn_sched_barrier(0);
+    four_step_phase1_zt<A, B, MZ, -1>(v, lg, twl);
 #pragma unroll
     for (int k1 = 0; k1 < A; ++k1) xb[k1 * XS + lg * CW + cl] = v[k1];
     wave_lds_sync();
@@ -282,9 +338,8 @@ __global__ __launch_bounds__(256, PD_COL_WAVES) void k_col_fast(ScanGeom g, cons
       const int k1 = lg + B * j;
       if (k1 < A) {
         rdft<B, -1>(w[j]);
-        if (colok)
 #pragma unroll
-          for (int k2 = 0; k2 < B; ++k2) acc[j][k2] = cadd(acc[j][k2], cmulc(w[j][k2], rv[j][k2]));
+        for (int k2 = 0; k2 < B; ++k2) acc[j][k2] = cadd(acc[j][k2], cmulc(w[j][k2], rv[j][k2]));
       }
     }
     if (t == T - 1) {
@@ -305,20 +360,22 @@ __global__ __launch_bounds__(256, PD_COL_WAVES) void k_col_fast(ScanGeom g, cons
       for (int k1 = 0; k1 < A; ++k1) x[k1] = xb[k1 * XS + lg * CW + cl];
       wave_lds_sync();
       rdft<A, +1>(x);
-      if (colok) {
-        const uint32_t sb = ((uint32_t)b * g.n0 * M + m0) * 8u;
+      const uint32_t sb = ((uint32_t)b * g.n0 * M + m0) * 8u;
 #pragma unroll
-        for (int m = 0; m < A; ++m) pd_st2(rs, (uint32_t)(lg + B * m) * M * 8u + vcol, sb, x[m]);
-      }
+      for (int m = 0; m < A; ++m) pd_st2(rs, (uint32_t)(lg + B * m) * M * 8u + vcol, sb, x[m]);
 #pragma unroll
       for (int j = 0; j < J; ++j)
 #pragma unroll
         for (int k = 0; k < B; ++k) acc[j][k] = make_float2(0.f, 0.f);
     }
-#if PD_COL_PREFETCH
-#pragma unroll
-    for (int m = 0; m < A; ++m) v[m] = vn[m];
-#endif
+  };
+  // two plane buffers in turn, so no copy waits on the prefetch
+  float2 va[A], vb[A];
+  load_planes(0, va);
+#pragma unroll 1
+  for (int it = 0; it < nit; it += 2) {
+    step(it, va, vb);
+    if (it + 1 < nit) step(it + 1, vb, va);
   }
 }
 
@@ -450,7 +507,7 @@ __global__ __launch_bounds__(512, PD_LIG_WAVES) void k_lig_plane_fwd_fast(ScanGe
         }
       }
       __syncthreads();
-      float2* dst = L1 + (((size_t)b * g.T + t) * g.p[0] + p) * (size_t)(N1 * H2);
+      float2* dst = L1 + (((size_t)b * g.T + t) * g.pp + (g.lo[0] - g.pl) + p) * (size_t)(N1 * H2);
       // plane stores through a buffer descriptor: 32-bit offsets instead of
       // one 64-bit address per stored value
       const __amdgpu_buffer_rsrc_t rd = pd_rsrc(dst, (uint32_t)(N1 * H2 * 8));

@@ -35,6 +35,12 @@ struct ScanGeom {
   int T;        // terms
   int lo[3];    // ligand box: cells [lo, lo+p) along each axis
   int p[3];
+  // Ligand spectra L1 hold pp planes per (rotation, term); plane q is grid
+  // plane pl + q (pl = lo[0]; planes q >= p[0] stay 0).  The column pass takes
+  // plane q as x position q and correlates it with the spectra of the receptor
+  // grid shifted by pl along x: shifting both grids leaves the correlation
+  // unchanged, and the ligand's nonzero positions start at 0.
+  int pl, pp;
   double h;
   double zero_h[3];  // centred-geometry zero_ - h per axis
   double zsc[3];     // centred-geometry cell-0 centre per axis
@@ -187,7 +193,7 @@ __global__ __launch_bounds__(512) void k_lig_plane_fwd(ScanGeom g, FftPlan pz, F
     fft_lds<-1>(buf + g.lo[1] * g.ls, pz, g.p[1], g.ls, 1);
     r2c_post(buf + g.lo[1] * g.ls, g.p[1], g.ls, g.m2, tw2);
     fft_lds<-1>(buf, py, g.h2, 1, g.ls);
-    float2* dst = L1 + (((size_t)b * g.T + t) * g.p[0] + p) * (size_t)g.M;
+    float2* dst = L1 + (((size_t)b * g.T + t) * g.pp + (g.lo[0] - g.pl) + p) * (size_t)g.M;
     for (int e = threadIdx.x; e < g.M; e += blockDim.x) {
       const int k1 = e / g.h2, k2 = e - k1 * g.h2;
       dst[e] = buf[k1 * g.ls + k2];
@@ -215,12 +221,11 @@ __global__ __launch_bounds__(256) void k_col(ScanGeom g, FftPlan px, const float
 #pragma unroll
     for (int q = 0; q < PD_COL_PER; ++q) acc[q] = make_float2(0.f, 0.f);
     for (int t = 0; t < g.T; ++t) {
-      const float2* src = L1 + ((size_t)b * g.T + t) * g.p[0] * (size_t)g.M + m0;
+      const float2* src = L1 + ((size_t)b * g.T + t) * g.pp * (size_t)g.M + m0;
       for (int e = threadIdx.x; e < nel; e += blockDim.x) {
         const int i0 = e / C, c = e - i0 * C;
-        const int p = i0 - g.lo[0];
         float2 v = make_float2(0.f, 0.f);
-        if (p >= 0 && p < g.p[0] && c < cw) v = src[(size_t)p * g.M + c];
+        if (i0 < g.pp && c < cw) v = src[(size_t)i0 * g.M + c];
         buf[e] = v;
       }
       __syncthreads();
@@ -498,14 +503,15 @@ __global__ __launch_bounds__(512) void k_rec_plane_fwd(ScanGeom g, FftPlan pz, F
 
 // Receptor column pass: FFT(x) only.  grid (ceil(M/C)), block 256, LDS n0*C.
 __global__ __launch_bounds__(256) void k_col_fwd(ScanGeom g, FftPlan px, const float2* __restrict__ in,
-                                                 float2* __restrict__ out, int C) {
+                                                 float2* __restrict__ out, int C, int shift) {
   extern __shared__ float2 buf[];
   const int m0 = blockIdx.x * C;
   const int cw = min(C, g.M - m0);
   const int nel = g.n0 * C;
   for (int e = threadIdx.x; e < nel; e += blockDim.x) {
     const int i0 = e / C, c = e - i0 * C;
-    buf[e] = c < cw ? in[(size_t)i0 * g.M + m0 + c] : make_float2(0.f, 0.f);
+    const int src = (i0 + shift) % g.n0;  // position i0 of the grid shifted by -shift
+    buf[e] = c < cw ? in[(size_t)src * g.M + m0 + c] : make_float2(0.f, 0.f);
   }
   __syncthreads();
   fft_lds<-1>(buf, px, C, 1, C);

@@ -55,7 +55,7 @@ __global__ __launch_bounds__(kSplitThreads) void k_lig_rows(ScanGeom g, FftPlan 
     __syncthreads();
     fft_lds<-1>(buf, pz, nr, g.ls, 1);
     r2c_post(buf, nr, g.ls, g.m2, tw2);
-    float2* dst = L1 + (((size_t)b * g.T + t) * g.p[0] + p) * (size_t)g.M + (size_t)r0 * g.h2;
+    float2* dst = L1 + (((size_t)b * g.T + t) * g.pp + (g.lo[0] - g.pl) + p) * (size_t)g.M + (size_t)r0 * g.h2;
     for (int e = threadIdx.x; e < nr * g.h2; e += blockDim.x) {
       const int rr = e / g.h2, k2 = e - rr * g.h2;
       dst[e] = buf[rr * g.ls + k2];
@@ -65,15 +65,17 @@ __global__ __launch_bounds__(kSplitThreads) void k_lig_rows(ScanGeom g, FftPlan 
 }
 
 // FFT along y of planes laid out [y][k2] (row stride h2), in place.  Plane q
-// starts at base + q * M; rows outside [ylo, yhi) are read as 0.
+// (q / grp groups of grp planes) starts at base + ((q / grp) * pitch + off +
+// q % grp) * M; rows outside [ylo, yhi) are read as 0.
 // grid (n_planes, ceil(h2 / kSplitCols)), LDS n1 * kSplitColStride complex.
 template <int DIR>
 __global__ __launch_bounds__(kSplitThreads) void k_cols_fft(ScanGeom g, FftPlan py, float2* __restrict__ base,
-                                                            int ylo, int yhi) {
+                                                            int grp, int pitch, int off, int ylo, int yhi) {
   extern __shared__ float2 buf[];
   const int c0 = blockIdx.y * kSplitCols;
   const int cw = min(kSplitCols, g.h2 - c0);
-  float2* pl = base + (size_t)blockIdx.x * g.M + c0;
+  const int q = blockIdx.x;
+  float2* pl = base + ((size_t)(q / grp) * pitch + off + q % grp) * g.M + c0;
   const int nel = g.n1 * kSplitCols;
   for (int e = threadIdx.x; e < nel; e += blockDim.x) {
     const int y = e / kSplitCols, c = e - y * kSplitCols;

@@ -102,6 +102,55 @@ def test_correlation_grid(oracle, n, h, n_rec, n_lig, T, flags):
             assert err <= tol
 
 
+# Ligand boxes of different widths select different column kernel variants:
+# the number MZ of x positions per lane that may hold ligand planes, and with
+# it the zero-tail first DFT stage (MZ dividing A) or the full DFT (otherwise).
+COLUMN_VARIANTS = [
+    ((32, 32, 32), 1.0, 200, 300, 2),       # A 4, MZ 4: full
+    ((64, 64, 64), 1.0, 400, 1500, 2),      # A 8, MZ 6: full DFT, 6 of 8 positions loaded
+    ((96, 96, 96), 1.0, 400, 3000, 2),      # A 12, MZ 9
+    ((128, 128, 128), 1.2, 1500, 1500, 4),  # A 16, MZ 8: zero-tail DFT (R 2)
+]
+
+
+@pytest.mark.parametrize("n,h,n_rec,n_lig,T", COLUMN_VARIANTS)
+def test_correlation_grid_column_variants(oracle, n, h, n_rec, n_lig, T):
+    print(f"case column variant n={n} n_lig={n_lig}", flush=True)
+    cx, rt, w, R = build_case(oracle, n, h, n_rec, n_lig, T, seed=12)
+    S = oracle.receptor_spectra(R)
+    rots = synth.rotation_set(2, seed=5)
+    tol = tolerance(R, w, n)
+    with make_scan(n, h, T, 4, cx, rt, w) as sc:
+        for r in rots:
+            q = oracle.rotate_points(r, cx.ligand.pos)
+            L = np.stack([oracle.project_ligand(n, h, q, w[t]) for t in range(T)])
+            ref = oracle.correlate(S, L)
+            got = sc.correlation_grid(r).astype(np.float64)
+            err = np.abs(got - ref).max()
+            print(f"  max|err|={err:.3e} tol={tol:.3e}", flush=True)
+            assert err <= tol
+
+
+def test_second_ligand_with_other_box(oracle):
+    """A second set_ligand with a wider box moves the L1 plane origin, which the
+    receptor spectra follow; the grids stay equal to the oracle's."""
+    n, h, T = (64, 64, 64), 1.0, 2
+    cx, rt, w, R = build_case(oracle, n, h, 400, 300, T, seed=12)
+    big = synth.make_complex(400, 1500, 12)
+    _, wb = synth.terms_for(big, T)
+    S = oracle.receptor_spectra(R)
+    r = synth.rotation_set(1, seed=6)[0]
+    tol = tolerance(R, w, n)
+    with make_scan(n, h, T, 4, cx, rt, w) as sc:
+        for lig, wl in ((cx.ligand.pos, w), (big.ligand.pos, wb), (cx.ligand.pos, w)):
+            sc.set_ligand(lig, wl)
+            q = oracle.rotate_points(r, lig)
+            L = np.stack([oracle.project_ligand(n, h, q, wl[t]) for t in range(T)])
+            err = np.abs(sc.correlation_grid(r).astype(np.float64) - oracle.correlate(S, L)).max()
+            print(f"  n_lig={len(lig)} max|err|={err:.3e} tol={tol:.3e}", flush=True)
+            assert err <= tolerance(R, wl, n)
+
+
 @pytest.mark.parametrize("n,h,n_rec,n_lig,T,flags", CASES_BOTH)
 def test_scan_peaks_match_oracle(oracle, n, h, n_rec, n_lig, T, flags):
     print(f"case scan peaks n={n} T={T} flags={flags}", flush=True)
