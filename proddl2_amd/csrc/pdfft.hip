@@ -485,8 +485,12 @@ pd_status run_batches(pd_scan* h, int n_rot, const double* d_rots, int rot0, Pea
     if (st != PD_OK) return st;
     {
       Timer tm(h, 4, ln.s);
-      hipLaunchKernelGGL(k_merge, dim3(nb), dim3(256), (size_t)g.n0 * h->K * 8, ln.s, g, h->K, ln.cval,
-                         ln.coff, h->d.max_val_corr, rot0 + b0, d_peaks + (size_t)b0 * h->K, d_nout + b0);
+      if (g.n0 <= 64 * kMergePlanes)
+        hipLaunchKernelGGL(k_merge_wave, dim3(nb), dim3(256), (size_t)g.n0 * h->K * 8, ln.s, g, h->K, ln.cval,
+                           ln.coff, h->d.max_val_corr, rot0 + b0, d_peaks + (size_t)b0 * h->K, d_nout + b0);
+      else
+        hipLaunchKernelGGL(k_merge, dim3(nb), dim3(256), (size_t)g.n0 * h->K * 8, ln.s, g, h->K, ln.cval,
+                           ln.coff, h->d.max_val_corr, rot0 + b0, d_peaks + (size_t)b0 * h->K, d_nout + b0);
     }
     PD_HIP(hipGetLastError());
   }
@@ -670,6 +674,7 @@ pd_status pd_scan_create(const pd_scan_desc* desc, pd_scan** out) {
   if ((st = set_lds((const void*)k_col, col_lds(h))) != PD_OK) return bad(st, pd_detail::last_error());
   if ((st = set_lds((const void*)k_col_fwd, col_lds(h))) != PD_OK) return bad(st, pd_detail::last_error());
   if ((st = set_lds((const void*)k_merge, g.n0 * h->K * 8)) != PD_OK) return bad(st, pd_detail::last_error());
+  if ((st = set_lds((const void*)k_merge_wave, g.n0 * h->K * 8)) != PD_OK) return bad(st, pd_detail::last_error());
   h->fast = h->split ? nullptr : find_fast(g.n0, g.n1, g.n2);
   h->fcol = find_fast_col(g.n0);
   if (h->fcol) {

@@ -163,6 +163,85 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// ---------------------------------------------------------------------------
+// Merge per rotation, one wave per rotation: the K best of the n0 per-plane
+// lists (each ascending in key order, at most K entries, offset -1 = empty).
+// The rotation's keys are staged in LDS; lane L owns planes L + 64 q and holds
+// each one's current head; K rounds of a wave minimum take the next best and
+// advance the list(s) it came from.  Values >= maxValCorr end the list, the
+// reference's queue bound (CorrelationProcessor::init, docking_fft.hpp:283-299).  grid (nrot), block 256:
+// all four waves stage the keys (many loads in flight), wave 0 merges.  LDS
+// n0 * K * 8 bytes; n0 <= 64 * kMergePlanes.
+// ---------------------------------------------------------------------------
+constexpr int kMergePlanes = 8;
+
+__global__ __launch_bounds__(256) void k_merge_wave(ScanGeom g, int K, const float* __restrict__ cval,
+                                                    const int* __restrict__ coff, double max_val, int rot0,
+                                                    PeakOut* __restrict__ peaks, int* __restrict__ n_out) {
+  extern __shared__ uint64_t keys[];
+  const int b = blockIdx.x, lane = threadIdx.x & 63;
+  const int nc = g.n0 * K;
+#pragma unroll 4
+  for (int e = threadIdx.x; e < nc; e += 256) {
+    float v = cval[(size_t)b * nc + e];
+    const int o = coff[(size_t)b * nc + e];
+    if (v != v) v = __int_as_float(0x7f800000);
+    keys[e] = o < 0 ? kNoKey : pd_key(v, o);
+  }
+  __syncthreads();
+  if (threadIdx.x >= 64) return;
+  int cur[kMergePlanes];
+  uint64_t hk[kMergePlanes];
+  uint64_t head = kNoKey;
+#pragma unroll
+  for (int q = 0; q < kMergePlanes; ++q) {
+    const int p = lane + 64 * q;
+    cur[q] = 0;
+    hk[q] = p < g.n0 ? keys[p * K] : kNoKey;
+    head = hk[q] < head ? hk[q] : head;
+  }
+  PeakOut* out = peaks + (size_t)b * K;
+  int count = 0;
+  for (int r = 0; r < K; ++r) {
+    const uint64_t m = wave_min_u64(head);  // the same in every lane
+    if (m == kNoKey) break;
+    const float v = pd_key_value(m);
+    if (!((double)v < max_val)) break;
+    if (lane == 0) {
+      PeakOut pk;
+      const int o = (int)(uint32_t)m;
+      pk.rot = rot0 + b;
+      pk.rank = r;
+      pk.offset = o;
+      const int x0 = o / (g.n1 * g.n2), rem = o - x0 * g.n1 * g.n2;
+      const int x1 = rem / g.n2, x2 = rem - x1 * g.n2;
+      const int x[3] = {x0, x1, x2}, n[3] = {g.n0, g.n1, g.n2};
+      for (int d = 0; d < 3; ++d) {
+        int y = -x[d];
+        if (y < -(n[d] / 2)) y += n[d];
+        pk.s[d] = y;
+      }
+      pk.value = v;
+      pk.reserved = 0.f;
+      pk.value64 = __longlong_as_double(0x7ff8000000000000LL);
+      out[r] = pk;
+    }
+    count = r + 1;
+    // advance every list whose head is this key (one entry per list)
+    head = kNoKey;
+#pragma unroll
+    for (int q = 0; q < kMergePlanes; ++q) {
+      if (hk[q] == m) {
+        const int p = lane + 64 * q;
+        ++cur[q];
+        hk[q] = cur[q] < K ? keys[p * K + cur[q]] : kNoKey;
+      }
+      head = hk[q] < head ? hk[q] : head;
+    }
+  }
+  if (lane == 0) n_out[b] = count;
+}
+
 // X[k] of the real row from the packed half-length spectrum Z (r2c post-step).
 __device__ __forceinline__ float2 r2c_point(float2 zk, float2 zm, float2 w) {
   const float2 E = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
