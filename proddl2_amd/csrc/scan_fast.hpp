@@ -460,6 +460,7 @@ __global__ __launch_bounds__(256, PD_COL_WAVES) void k_col_fast(ScanGeom g, cons
 
 // ---------------------------------------------------------------------------
 // Ligand plane pass.  grid (p0, nrot), block 512, LDS N1 * LS complex.
+constexpr int kLigList = 1024;  // atoms of one plane kept in the block's list
 // ---------------------------------------------------------------------------
 template <int N1, int N2>
 __global__ __launch_bounds__(512, PD_LIG_WAVES) void k_lig_plane_fwd_fast(ScanGeom g, const float2* __restrict__ twN1,
@@ -484,16 +485,43 @@ __global__ __launch_bounds__(512, PD_LIG_WAVES) void k_lig_plane_fwd_fast(ScanGe
   const int plane = N1 * N2;
   const int* cb = cells + (size_t)b * n_atoms;
   const int gz = tid / LZ, rz = tid % LZ, gy = tid / LY, cy = tid % LY;
+  // The plane's atoms, found once for all terms: atom index and position in
+  // the plane buffer (as a float index).  More than kLigList atoms in one
+  // plane fall back to scanning every atom per term.
+  __shared__ int s_cnt;
+  __shared__ int s_atom[kLigList];
+  __shared__ int s_pos[kLigList];
+  if (tid == 0) s_cnt = 0;
+  __syncthreads();
+#pragma unroll 4
+  for (int j = tid; j < n_atoms; j += NT) {
+    const int off = cb[j];
+    if (off >= 0 && off / plane == i0) {
+      const int r = off - i0 * plane;
+      const int i1 = r / N2, i2 = r - i1 * N2;
+      const int k = atomicAdd(&s_cnt, 1);
+      if (k < kLigList) {
+        s_atom[k] = j;
+        s_pos[k] = i1 * 2 * LS + i2;
+      }
+    }
+  }
+  __syncthreads();
+  const int cnt = s_cnt;
   for (int t = 0; t < g.T; ++t) {
     for (int e = tid; e < p1 * M2; e += NT) buf[(lo1 + e / M2) * LS + e % M2] = make_float2(0.f, 0.f);
     __syncthreads();
     const float* wt = w + (size_t)t * n_atoms;
-    for (int j = tid; j < n_atoms; j += NT) {
-      const int off = cb[j];
-      if (off >= 0 && off / plane == i0) {
-        const int r = off - i0 * plane;
-        const int i1 = r / N2, i2 = r - i1 * N2;
-        atomicAdd(&fb[i1 * 2 * LS + i2], wt[j]);
+    if (cnt <= kLigList) {
+      for (int k = tid; k < cnt; k += NT) atomicAdd(&fb[s_pos[k]], wt[s_atom[k]]);
+    } else {
+      for (int j = tid; j < n_atoms; j += NT) {
+        const int off = cb[j];
+        if (off >= 0 && off / plane == i0) {
+          const int r = off - i0 * plane;
+          const int i1 = r / N2, i2 = r - i1 * N2;
+          atomicAdd(&fb[i1 * 2 * LS + i2], wt[j]);
+        }
       }
     }
     __syncthreads();
