@@ -94,19 +94,37 @@ def pruned_search(sc, cx, d_rots, ex_peaks, t_exhaustive_s, log):
     }
 
 
-def cpu_baseline(cx, rt, w, wl, rots, threads, log, per_thread=32):
+def host_cpu():
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"model": model, "nproc": os.cpu_count()}
+
+
+def cpu_baseline(cx, rt, w, wl, rots, threads, log):
     from oracle import pdoracle
     n, h, T, K = wl["n"], wl["h"], wl["n_terms"], wl["K"]
     t0 = time.perf_counter()
     R = np.stack([pdoracle.project_receptor(n, h, cx.receptor.pos, k, p, cut, md) for (k, cut, md, p) in rt])
     S = pdoracle.receptor_spectra(R)
     log(f"cpu baseline: receptor precompute {time.perf_counter() - t0:.1f} s")
-    # a bounded sample: per_thread rotations on each thread (about 10-15 s at 128^3 x 4)
+    # a bounded sample: 32 rotations per thread at 128^3 (about 12 s), fewer on larger grids
+    per_thread = max(2, round(32 * 128 ** 3 / float(np.prod(n))))
     sample = rots[: threads * per_thread]
     t0 = time.perf_counter()
     pdoracle.scan(n, h, S, cx.ligand.pos, w, sample, K=K, max_val=1e16, n_threads=threads)
     dt = time.perf_counter() - t0
     log(f"cpu baseline: {len(sample)} rotations on {threads} threads in {dt:.1f} s")
+    one = rots[: max(2, per_thread // 4)]
+    t1 = time.perf_counter()
+    pdoracle.scan(n, h, S, cx.ligand.pos, w, one, K=K, max_val=1e16, n_threads=1)
+    dt1 = time.perf_counter() - t1
+    log(f"cpu baseline: {len(one)} rotations on 1 thread in {dt1:.1f} s")
     return {
         "value": len(sample) / dt,
         "unit": "rotations/s",
@@ -115,7 +133,26 @@ def cpu_baseline(cx, rt, w, wl, rots, threads, log, per_thread=32):
         "sample": f"{len(sample)} rotations of the same workload ({per_thread} per thread), FP64 oracle "
                   f"restatement of the reference worker loop with its own FFT (not FFTW), "
                   f"receptor precompute excluded",
+        "single_thread": {"value": len(one) / dt1, "unit": "rotations/s", "sample": f"{len(one)} rotations"},
+        "host": host_cpu(),
     }
+
+
+def copy_bandwidth(torch, dev, reps=5):
+    """Measured device-to-device copy rate of 1 GiB (bytes read + written per second):
+    what HBM sustains for a plain stream next to the 8 TB/s specification."""
+    src = torch.ones(1 << 28, dtype=torch.float32, device=dev)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del src, dst
+    return {"GBs": 2 * (1 << 30) / (ms * 1e-3) / 1e9, "bytes_per_copy": 2 * (1 << 30), "ms_per_copy": ms}
 
 
 def main() -> int:
@@ -225,7 +262,8 @@ def main() -> int:
     # FP64 rescoring of the global top poses (outside the timed region).
     peaks = d_peaks.cpu().numpy().view(pdfft.PEAK_DTYPE).reshape(n_rot, K)
     flat = peaks.reshape(-1)
-    n_top = min(1000, flat.size)
+    # global top poses rescored in FP64: 1,000 at 128^3, 2,000 on larger grids
+    n_top = min(1000 if np.prod(n) <= 128 ** 3 else 2000, flat.size)
     top = flat[np.argsort(flat["value"], kind="stable")[:n_top]]
     t0 = time.perf_counter()
     e64 = sc.rescore64(rots[top["rot"]], top["s"])
@@ -236,6 +274,8 @@ def main() -> int:
     if not args.no_pruned:
         pruned = pruned_search(sc, cx, d_rots, peaks, ms_per_step / 1e3, log)
 
+    hbm_copy = copy_bandwidth(torch, dev)
+    log(f"device copy: {hbm_copy['GBs']:.0f} GB/s (read + write)")
     threads = args.cpu_threads or min(16, os.cpu_count() or 1)
     cpu = None
     if not args.no_cpu_baseline:
@@ -279,6 +319,7 @@ def main() -> int:
             "fft_gflops": rate * (T + 1) * 5 * np.prod(n) * np.log2(np.prod(n)) / 1e9,
         },
         "kernels": kern,
+        "hbm_copy": hbm_copy,
         "receptor_precompute_ms": precompute_s * 1e3,
         "rescore64": {"poses": int(n_top), "ms": rescore_ms,
                       "max_rel_err_fp32": float(rel.max()), "median_rel_err_fp32": float(np.median(rel))},

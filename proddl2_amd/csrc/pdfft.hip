@@ -89,6 +89,26 @@ struct FastRows {
 const FastRows kFastRows[] = {PD_FROWS(48), PD_FROWS(64), PD_FROWS(96), PD_FROWS(128), PD_FROWS(256)};
 #undef PD_FROWS
 
+// y passes of the split path specialised on N1.
+using YColFn = void (*)(ScanGeom, const float2*, float2*, int, int, int, int, int);
+struct FastYCols {
+  int n1;
+  int cols;  // columns per block
+  int lds;   // bytes: N1 rows of cols + 1 complex
+  YColFn fwd, inv;
+};
+#define PD_FYCOL(N) \
+  FastYCols { N, 256 / Split<N>::B, N * (256 / Split<N>::B + 1) * 8, k_cols_fast<N, -1>, k_cols_fast<N, +1> }
+const FastYCols kFastYCols[] = {PD_FYCOL(32),  PD_FYCOL(48),  PD_FYCOL(64),  PD_FYCOL(96), PD_FYCOL(120),
+                                PD_FYCOL(128), PD_FYCOL(144), PD_FYCOL(160), PD_FYCOL(256)};
+#undef PD_FYCOL
+
+const FastYCols* find_fast_ycols(int n1) {
+  for (const FastYCols& f : kFastYCols)
+    if (f.n1 == n1) return &f;
+  return nullptr;
+}
+
 const FastRows* find_fast_rows(int n2) {
   for (const FastRows& f : kFastRows)
     if (f.n2 == n2) return &f;
@@ -205,6 +225,7 @@ struct pd_scan {
   const FastSet* fast = nullptr;
   const FastCol* fcol = nullptr;  // column kernel specialised on N0, if any
   const FastRows* frows = nullptr;  // split inverse rows specialised on N2, if any
+  const FastYCols* fycol = nullptr;  // split y passes specialised on N1, if any
   bool split = false;  // plane passes as row + column kernels (scan_split.hpp)
   HostPlan hx, hy, hz;
   FftPlan px{}, py{}, pz{};
@@ -349,8 +370,12 @@ pd_status receptor_spectra(pd_scan* h) {
     if (h->split) {
       hipLaunchKernelGGL(k_rec_rows, dim3(g.n0, (g.n1 + kSplitRows - 1) / kSplitRows), dim3(kSplitThreads),
                          split_row_lds(h), h->stream, g, h->pz, h->tw2, r32 + (size_t)t * h->N, tmp);
-      hipLaunchKernelGGL(k_cols_fft<-1>, dim3(g.n0, (g.h2 + kSplitCols - 1) / kSplitCols), dim3(kSplitThreads),
-                         split_col_lds(h), h->stream, g, h->py, tmp, g.n0, g.n0, 0, 0, g.n1);
+      if (h->fycol)
+        hipLaunchKernelGGL(h->fycol->fwd, dim3(g.n0, (g.h2 + h->fycol->cols - 1) / h->fycol->cols), dim3(256),
+                           h->fycol->lds, h->stream, g, h->tw_y, tmp, g.n0, g.n0, 0, 0, g.n1);
+      else
+        hipLaunchKernelGGL(k_cols_fft<-1>, dim3(g.n0, (g.h2 + kSplitCols - 1) / kSplitCols), dim3(kSplitThreads),
+                           split_col_lds(h), h->stream, g, h->py, tmp, g.n0, g.n0, 0, 0, g.n1);
     } else {
       hipLaunchKernelGGL(k_rec_plane_fwd, dim3(g.n0), dim3(kPlaneThreads), plane_lds(h), h->stream, g,
                          h->pz, h->py, h->tw2, r32 + (size_t)t * h->N, tmp);
@@ -396,9 +421,14 @@ pd_status launch_fft_path(pd_scan* h, const pd_scan::Lane& ln, int nb, const dou
       hipLaunchKernelGGL(k_lig_rows, dim3(g.p[0], nb, (g.p[1] + kSplitRows - 1) / kSplitRows),
                          dim3(kSplitThreads), split_row_lds(h), stream, g, h->pz, h->tw2, ln.cells, h->n_lig,
                          h->w32, ln.L1);
-      hipLaunchKernelGGL(k_cols_fft<-1>, dim3(nb * g.T * g.p[0], (g.h2 + kSplitCols - 1) / kSplitCols),
-                         dim3(kSplitThreads), split_col_lds(h), stream, g, h->py, ln.L1, g.p[0], g.pp,
-                         g.lo[0] - g.pl, g.lo[1], g.lo[1] + g.p[1]);
+      if (h->fycol)
+        hipLaunchKernelGGL(h->fycol->fwd, dim3(nb * g.T * g.p[0], (g.h2 + h->fycol->cols - 1) / h->fycol->cols),
+                           dim3(256), h->fycol->lds, stream, g, h->tw_y, ln.L1, g.p[0], g.pp, g.lo[0] - g.pl,
+                           g.lo[1], g.lo[1] + g.p[1]);
+      else
+        hipLaunchKernelGGL(k_cols_fft<-1>, dim3(nb * g.T * g.p[0], (g.h2 + kSplitCols - 1) / kSplitCols),
+                           dim3(kSplitThreads), split_col_lds(h), stream, g, h->py, ln.L1, g.p[0], g.pp,
+                           g.lo[0] - g.pl, g.lo[1], g.lo[1] + g.p[1]);
     } else if (f)
       hipLaunchKernelGGL(f->lig, dim3(g.p[0], nb), dim3(kPlaneThreads), plane_lds(h), stream, g,
                          h->tw_y, h->tw_z, h->tw2, ln.cells, h->n_lig, h->w32, ln.L1);
@@ -434,8 +464,12 @@ pd_status launch_fft_path(pd_scan* h, const pd_scan::Lane& ln, int nb, const dou
   {
     Timer tm(h, 3, stream);
     if (h->split) {
-      hipLaunchKernelGGL(k_cols_fft<+1>, dim3(nb * g.n0, (g.h2 + kSplitCols - 1) / kSplitCols), dim3(kSplitThreads),
-                         split_col_lds(h), stream, g, h->py, ln.S1, g.n0, g.n0, 0, 0, g.n1);
+      if (h->fycol)
+        hipLaunchKernelGGL(h->fycol->inv, dim3(nb * g.n0, (g.h2 + h->fycol->cols - 1) / h->fycol->cols), dim3(256),
+                           h->fycol->lds, stream, g, h->tw_y, ln.S1, g.n0, g.n0, 0, 0, g.n1);
+      else
+        hipLaunchKernelGGL(k_cols_fft<+1>, dim3(nb * g.n0, (g.h2 + kSplitCols - 1) / kSplitCols),
+                           dim3(kSplitThreads), split_col_lds(h), stream, g, h->py, ln.S1, g.n0, g.n0, 0, 0, g.n1);
       if (h->frows)
         hipLaunchKernelGGL(h->frows->fn, dim3(nb, g.n0), dim3(512), h->frows->lds, stream, g, h->tw_z, h->tw2, ln.S1,
                            scale, h->K, ln.cval, ln.coff, grid_out, ln.thr);
@@ -685,6 +719,10 @@ pd_status pd_scan_create(const pd_scan_desc* desc, pd_scan** out) {
   if (h->split) {
     h->frows = find_fast_rows(g.n2);
     if (h->frows && (st = set_lds((const void*)h->frows->fn, h->frows->lds)) != PD_OK)
+      return bad(st, pd_detail::last_error());
+    h->fycol = find_fast_ycols(g.n1);
+    if (h->fycol && ((st = set_lds((const void*)h->fycol->fwd, h->fycol->lds)) != PD_OK ||
+                     (st = set_lds((const void*)h->fycol->inv, h->fycol->lds)) != PD_OK))
       return bad(st, pd_detail::last_error());
   }
   if (h->fast) {

@@ -91,6 +91,65 @@ __global__ __launch_bounds__(kSplitThreads) void k_cols_fft(ScanGeom g, FftPlan 
   }
 }
 
+// The same y FFT with N1 known at compile time: a tile of N1 rows x CB
+// columns (CB = 256 / B, 128-byte row pieces) is read into LDS with coalesced
+// loads, each column is transformed by B lanes with the register four-step
+// FFT (one exchange through the tile), and the tile is written back
+// coalesced.  grid (n_planes, ceil(h2 / CB)), block 256, LDS N1 * (CB + 1)
+// complex.  Same plane addressing and row window as k_cols_fft.
+template <int N1, int DIR>
+__global__ __launch_bounds__(256) void k_cols_fast(ScanGeom g, const float2* __restrict__ twN1,
+                                                   float2* __restrict__ base, int grp, int pitch, int off, int ylo,
+                                                   int yhi) {
+  using S = Split<N1>;
+  constexpr int A = S::A, B = S::B, J = S::J, CB = 256 / B, CBS = CB + 1;
+  static_assert(A * B == N1 && 256 % B == 0, "column split");
+  extern __shared__ float2 tile[];
+  const int c0 = blockIdx.y * CB;
+  const int cw = min(CB, g.h2 - c0);
+  const int q = blockIdx.x;
+  float2* pl = base + ((size_t)(q / grp) * pitch + off + q % grp) * g.M + c0;
+  for (int e = threadIdx.x; e < N1 * CB; e += 256) {
+    const int y = e / CB, c = e % CB;
+    float2 v = make_float2(0.f, 0.f);
+    if (c < cw && y >= ylo && y < yhi) v = pl[(size_t)y * g.h2 + c];
+    tile[y * CBS + c] = v;
+  }
+  __syncthreads();
+  const int cl = threadIdx.x % CB, lg = threadIdx.x / CB;
+  float2 x[A];
+#pragma unroll
+  for (int m = 0; m < A; ++m) x[m] = tile[(lg + B * m) * CBS + cl];
+  four_step_phase1<A, B, DIR>(x, lg, twN1);
+  __syncthreads();
+#pragma unroll
+  for (int k1 = 0; k1 < A; ++k1) tile[(k1 * B + lg) * CBS + cl] = x[k1];
+  __syncthreads();
+  float2 w[J][B];
+#pragma unroll
+  for (int j = 0; j < J; ++j) {
+    const int k1 = lg + B * j;
+    if (k1 < A)
+#pragma unroll
+      for (int qq = 0; qq < B; ++qq) w[j][qq] = tile[(k1 * B + qq) * CBS + cl];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < J; ++j) {
+    const int k1 = lg + B * j;
+    if (k1 < A) {
+      rdft<B, DIR>(w[j]);
+#pragma unroll
+      for (int k2 = 0; k2 < B; ++k2) tile[(k1 + A * k2) * CBS + cl] = w[j][k2];
+    }
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < N1 * CB; e += 256) {
+    const int y = e / CB, c = e % CB;
+    if (c < cw) pl[(size_t)y * g.h2 + c] = tile[y * CBS + c];
+  }
+}
+
 // Receptor rows: dense real rows -> r2c(z).  grid (n0, ceil(n1 / kSplitRows)).
 __global__ __launch_bounds__(kSplitThreads) void k_rec_rows(ScanGeom g, FftPlan pz, const float2* __restrict__ tw2,
                                                             const float* __restrict__ R,
