@@ -670,7 +670,7 @@ __global__ __launch_bounds__(512, PD_PLANE_WAVES) void k_plane_inv_topk_fast(Sca
   using SZ = Split<M2>;
   constexpr int A1 = SY::A, B1 = SY::B, J1 = SY::J;
   constexpr int A2 = SZ::A, B2 = SZ::B, J2 = SZ::J;
-  constexpr int LY = NT / B1, RY = (H2 + LY - 1) / LY;
+  constexpr int LY = NT / B1, RY = (M2 + LY - 1) / LY;  // y columns: M2 (0 and M2 packed)
   constexpr int LZ = NT / B2, RZ = (N1 + LZ - 1) / LZ;
   extern __shared__ float2 buf[];
   const int tid = threadIdx.x;
@@ -684,21 +684,39 @@ __global__ __launch_bounds__(512, PD_PLANE_WAVES) void k_plane_inv_topk_fast(Sca
   // atomics do not update.
   __shared__ uint32_t s_thr;
   if (tid == 0) s_thr = thr ? __hip_atomic_load(thr + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0xFFFFFFFFu;
-  // y pass: backward FFT along y of every column
+  // y pass: backward FFT along y of every column.  Columns 0 and M2 are
+  // Hermitian along y (their transforms are real): they travel packed as one
+  // complex column X[.][0] + i X[.][M2] in column slot 0, so the pass covers M2
+  // columns and the z pass unpacks real and imaginary parts.  Every lane loads
+  // column M2 of its rows with an ordinary vector load issued with the others:
+  // a divergent, scalar or LDS-staged load costs an extra memory round trip
+  // before the transform.  The empty asm consumes those values in every lane,
+  // so the compiler cannot sink the loads into the column-0 branch.
   {
     const int gy = tid / LY, cy = tid % LY;
     float2 v[RY][A1];
 #pragma unroll
     for (int r = 0; r < RY; ++r) {
       const int col = r * LY + cy;
-      if (col < H2)
+      if (col < M2) {
+        float2 ym[A1];
 #pragma unroll
-        for (int m = 0; m < A1; ++m) v[r][m] = src[(gy + B1 * m) * H2 + col];
+        for (int m = 0; m < A1; ++m) {
+          const int row = gy + B1 * m;
+          v[r][m] = src[row * H2 + col];
+          ym[m] = src[row * H2 + M2];
+        }
+#pragma unroll
+        for (int m = 0; m < A1; ++m) {
+          asm volatile("" ::"v"(ym[m].x), "v"(ym[m].y));
+          if (col == 0) v[r][m] = make_float2(v[r][m].x - ym[m].y, v[r][m].y + ym[m].x);
+        }
+      }
     }
 #pragma unroll
     for (int r = 0; r < RY; ++r) {
       const int col = r * LY + cy;
-      if (col < H2) {
+      if (col < M2) {
         four_step_phase1<A1, B1, +1>(v[r], gy, twN1);
 #pragma unroll
         for (int k1 = 0; k1 < A1; ++k1) buf[(k1 * B1 + gy) * LS + col] = v[r][k1];
@@ -712,7 +730,7 @@ __global__ __launch_bounds__(512, PD_PLANE_WAVES) void k_plane_inv_topk_fast(Sca
 #pragma unroll
       for (int j = 0; j < J1; ++j) {
         const int k1 = gy + B1 * j;
-        if (col < H2 && k1 < A1)
+        if (col < M2 && k1 < A1)
 #pragma unroll
           for (int q = 0; q < B1; ++q) w[r][j][q] = buf[(k1 * B1 + q) * LS + col];
       }
@@ -724,7 +742,7 @@ __global__ __launch_bounds__(512, PD_PLANE_WAVES) void k_plane_inv_topk_fast(Sca
 #pragma unroll
       for (int j = 0; j < J1; ++j) {
         const int k1 = gy + B1 * j;
-        if (col < H2 && k1 < A1) {
+        if (col < M2 && k1 < A1) {
           rdft<B1, +1>(w[r][j]);
 #pragma unroll
           for (int k2 = 0; k2 < B1; ++k2) buf[(k1 + A1 * k2) * LS + col] = w[r][j][k2];
@@ -745,8 +763,13 @@ __global__ __launch_bounds__(512, PD_PLANE_WAVES) void k_plane_inv_topk_fast(Sca
 #pragma unroll
         for (int m = 0; m < A2; ++m) {
           const int k = gz + B2 * m;
-          float2 xk = buf[row * LS + k], xm = buf[row * LS + M2 - k];
-          if (k == 0) { xk.y = 0.f; xm.y = 0.f; }
+          float2 xk = buf[row * LS + k], xm;
+          if (k == 0) {  // packed column: X[0] = Re, X[M2] = Im (both real)
+            xm = make_float2(xk.y, 0.f);
+            xk.y = 0.f;
+          } else {
+            xm = buf[row * LS + M2 - k];
+          }
           z[r][m] = c2r_point(xk, xm, tw2[k]);
         }
     }
