@@ -485,6 +485,10 @@ __global__ __launch_bounds__(512, PD_LIG_WAVES) void k_lig_plane_fwd_fast(ScanGe
   const int plane = N1 * N2;
   const int* cb = cells + (size_t)b * n_atoms;
   const int gz = tid / LZ, rz = tid % LZ, gy = tid / LY, cy = tid % LY;
+  // LZ and LY are multiples of 64, so gz and gy are wave-uniform: as uniform
+  // values they make the twiddle reads scalar loads, off the vector memory path.
+  static_assert(LZ % 64 == 0 && LY % 64 == 0, "one lane group per wave");
+  const int gzu = __builtin_amdgcn_readfirstlane(gz), gyu = __builtin_amdgcn_readfirstlane(gy);
   // The plane's atoms, found once for all terms: atom index and position in
   // the plane buffer (as a float index).  More than kLigList atoms in one
   // plane fall back to scanning every atom per term.
@@ -543,7 +547,7 @@ __global__ __launch_bounds__(512, PD_LIG_WAVES) void k_lig_plane_fwd_fast(ScanGe
         const int rr = r * LZ + rz;
         if (rr < p1) {
           const int row = lo1 + rr;
-          four_step_phase1<A2, B2, -1>(z[r], gz, twM2);
+          four_step_phase1<A2, B2, -1>(z[r], gzu, twM2);
 #pragma unroll
           for (int k1 = 0; k1 < A2; ++k1) buf[row * LS + k1 * B2 + gz] = z[r][k1];
         }
@@ -608,7 +612,7 @@ __global__ __launch_bounds__(512, PD_LIG_WAVES) void k_lig_plane_fwd_fast(ScanGe
       for (int r = 0; r < RY; ++r) {
         const int col = r * LY + cy;
         if (col < M2) {
-          four_step_phase1<A1, B1, -1>(v[r], gy, twN1);
+          four_step_phase1<A1, B1, -1>(v[r], gyu, twN1);
 #pragma unroll
           for (int k1 = 0; k1 < A1; ++k1) buf[(k1 * B1 + gy) * LS + col] = v[r][k1];
         }
@@ -681,9 +685,9 @@ __global__ __launch_bounds__(512, PD_PLANE_WAVES) void k_plane_inv_topk_fast(Sca
   // The rotation's selection threshold, read once per block before the
   // transforms (an older value is larger, hence still a valid bound).  The
   // read is device-coherent: each XCD has its own L2, which the other XCDs'
-  // atomics do not update.
+  // atomics do not update.  It is issued after the plane's loads, so that its
+  // latency overlaps theirs; the barriers of the y pass publish it.
   __shared__ uint32_t s_thr;
-  if (tid == 0) s_thr = thr ? __hip_atomic_load(thr + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0xFFFFFFFFu;
   // y pass: backward FFT along y of every column.  Columns 0 and M2 are
   // Hermitian along y (their transforms are real): they travel packed as one
   // complex column X[.][0] + i X[.][M2] in column slot 0, so the pass covers M2
@@ -694,6 +698,9 @@ __global__ __launch_bounds__(512, PD_PLANE_WAVES) void k_plane_inv_topk_fast(Sca
   // so the compiler cannot sink the loads into the column-0 branch.
   {
     const int gy = tid / LY, cy = tid % LY;
+    // LY is a multiple of 64: gy is wave-uniform, and so are the twiddle reads
+    static_assert(LY % 64 == 0, "one lane group per wave");
+    const int gyu = __builtin_amdgcn_readfirstlane(gy);
     float2 v[RY][A1];
 #pragma unroll
     for (int r = 0; r < RY; ++r) {
@@ -713,11 +720,12 @@ __global__ __launch_bounds__(512, PD_PLANE_WAVES) void k_plane_inv_topk_fast(Sca
         }
       }
     }
+    if (tid == 0) s_thr = thr ? __hip_atomic_load(thr + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0xFFFFFFFFu;
 #pragma unroll
     for (int r = 0; r < RY; ++r) {
       const int col = r * LY + cy;
       if (col < M2) {
-        four_step_phase1<A1, B1, +1>(v[r], gy, twN1);
+        four_step_phase1<A1, B1, +1>(v[r], gyu, twN1);
 #pragma unroll
         for (int k1 = 0; k1 < A1; ++k1) buf[(k1 * B1 + gy) * LS + col] = v[r][k1];
       }
@@ -753,6 +761,8 @@ __global__ __launch_bounds__(512, PD_PLANE_WAVES) void k_plane_inv_topk_fast(Sca
   }
   // z pass: c2r pre-step on the fly, backward half-length FFT of every row
   const int gz = tid / LZ, rz = tid % LZ;
+  static_assert(LZ % 64 == 0, "one lane group per wave");
+  const int gzu = __builtin_amdgcn_readfirstlane(gz);  // wave-uniform twiddle reads
   float2 u[RZ][J2][B2];
   {
     float2 z[RZ][A2];
@@ -762,7 +772,7 @@ __global__ __launch_bounds__(512, PD_PLANE_WAVES) void k_plane_inv_topk_fast(Sca
       if (row < N1)
 #pragma unroll
         for (int m = 0; m < A2; ++m) {
-          const int k = gz + B2 * m;
+          const int k = gz + B2 * m, ku = gzu + B2 * m;
           float2 xk = buf[row * LS + k], xm;
           if (k == 0) {  // packed column: X[0] = Re, X[M2] = Im (both real)
             xm = make_float2(xk.y, 0.f);
@@ -770,7 +780,7 @@ __global__ __launch_bounds__(512, PD_PLANE_WAVES) void k_plane_inv_topk_fast(Sca
           } else {
             xm = buf[row * LS + M2 - k];
           }
-          z[r][m] = c2r_point(xk, xm, tw2[k]);
+          z[r][m] = c2r_point(xk, xm, tw2[ku]);
         }
     }
     __syncthreads();
@@ -778,7 +788,7 @@ __global__ __launch_bounds__(512, PD_PLANE_WAVES) void k_plane_inv_topk_fast(Sca
     for (int r = 0; r < RZ; ++r) {
       const int row = r * LZ + rz;
       if (row < N1) {
-        four_step_phase1<A2, B2, +1>(z[r], gz, twM2);
+        four_step_phase1<A2, B2, +1>(z[r], gzu, twM2);
 #pragma unroll
         for (int k1 = 0; k1 < A2; ++k1) buf[row * LS + k1 * B2 + gz] = z[r][k1];
       }

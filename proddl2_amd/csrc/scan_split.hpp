@@ -293,9 +293,12 @@ __global__ __launch_bounds__(512) void k_inv_rows_topk_fast(ScanGeom g, const fl
   const float INF = __int_as_float(0x7f800000);
   const int plane = N1 * N2, base = i0 * plane;
   const int gz = tid / LZ, rz = tid % LZ;
+  static_assert(LZ % 64 == 0, "one lane group per wave");
+  const int gzu = __builtin_amdgcn_readfirstlane(gz);  // wave-uniform twiddle reads
+  // best and s_thr are first read after the barriers of the first chunk; the
+  // threshold read goes out after that chunk's loads so its latency overlaps
+  // theirs.
   if (tid < 32) best[tid] = kNoKey;
-  if (tid == 0) s_thr = thr ? __hip_atomic_load(thr + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0xFFFFFFFFu;
-  __syncthreads();
   for (int r0 = 0; r0 < N1; r0 += LZ) {
     const int row = r0 + rz;
     const bool rok = row < N1;
@@ -309,9 +312,11 @@ __global__ __launch_bounds__(512) void k_inv_rows_topk_fast(ScanGeom g, const fl
         xm = src[row * H2 + M2 - k];
       }
       if (k == 0) { xk.y = 0.f; xm.y = 0.f; }
-      z[m] = c2r_point(xk, xm, tw2[k]);
+      z[m] = c2r_point(xk, xm, tw2[gzu + B2 * m]);
     }
-    four_step_phase1<A2, B2, +1>(z, gz, twM2);
+    if (r0 == 0 && tid == 0)
+      s_thr = thr ? __hip_atomic_load(thr + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0xFFFFFFFFu;
+    four_step_phase1<A2, B2, +1>(z, gzu, twM2);
     if (rok)
 #pragma unroll
       for (int k1 = 0; k1 < A2; ++k1) buf[rz * LS + k1 * B2 + gz] = z[k1];
