@@ -485,10 +485,12 @@ __global__ __launch_bounds__(512, PD_LIG_WAVES) void k_lig_plane_fwd_fast(ScanGe
   const int plane = N1 * N2;
   const int* cb = cells + (size_t)b * n_atoms;
   const int gz = tid / LZ, rz = tid % LZ, gy = tid / LY, cy = tid % LY;
-  // LZ and LY are multiples of 64, so gz and gy are wave-uniform: as uniform
-  // values they make the twiddle reads scalar loads, off the vector memory path.
-  static_assert(LZ % 64 == 0 && LY % 64 == 0, "one lane group per wave");
-  const int gzu = __builtin_amdgcn_readfirstlane(gz), gyu = __builtin_amdgcn_readfirstlane(gy);
+  // LZ is a multiple of 64, so gz is wave-uniform: as a uniform value it makes
+  // the z pass's twiddle reads scalar loads, off the vector memory path.  (In
+  // the y pass, whose loads come from LDS, scalar twiddle reads measured
+  // slower: they share the LDS wait counter.)
+  static_assert(LZ % 64 == 0, "one lane group per wave");
+  const int gzu = __builtin_amdgcn_readfirstlane(gz);
   // The plane's atoms, found once for all terms: atom index and position in
   // the plane buffer (as a float index).  More than kLigList atoms in one
   // plane fall back to scanning every atom per term.
@@ -612,7 +614,7 @@ __global__ __launch_bounds__(512, PD_LIG_WAVES) void k_lig_plane_fwd_fast(ScanGe
       for (int r = 0; r < RY; ++r) {
         const int col = r * LY + cy;
         if (col < M2) {
-          four_step_phase1<A1, B1, -1>(v[r], gyu, twN1);
+          four_step_phase1<A1, B1, -1>(v[r], gy, twN1);
 #pragma unroll
           for (int k1 = 0; k1 < A1; ++k1) buf[(k1 * B1 + gy) * LS + col] = v[r][k1];
         }
