@@ -109,6 +109,14 @@ __global__ __launch_bounds__(256) void k_cols_fast(ScanGeom g, const float2* __r
   const int cw = min(CB, g.h2 - c0);
   const int q = blockIdx.x;
   float2* pl = base + ((size_t)(q / grp) * pitch + off + q % grp) * g.M + c0;
+  const int cl = threadIdx.x % CB, lg = threadIdx.x / CB;
+  // the lane's twiddles W_N1^{lg k1}, loaded with the tile rather than after it
+  float2 tw[A];
+#pragma unroll
+  for (int k1 = 1; k1 < A; ++k1) {
+    tw[k1] = twN1[lg * k1];
+    if (DIR > 0) tw[k1].y = -tw[k1].y;
+  }
   for (int e = threadIdx.x; e < N1 * CB; e += 256) {
     const int y = e / CB, c = e % CB;
     float2 v = make_float2(0.f, 0.f);
@@ -116,11 +124,12 @@ __global__ __launch_bounds__(256) void k_cols_fast(ScanGeom g, const float2* __r
     tile[y * CBS + c] = v;
   }
   __syncthreads();
-  const int cl = threadIdx.x % CB, lg = threadIdx.x / CB;
   float2 x[A];
 #pragma unroll
   for (int m = 0; m < A; ++m) x[m] = tile[(lg + B * m) * CBS + cl];
-  four_step_phase1<A, B, DIR>(x, lg, twN1);
+  rdft<A, DIR>(x);
+#pragma unroll
+  for (int k1 = 1; k1 < A; ++k1) x[k1] = cmul(x[k1], tw[k1]);
   __syncthreads();
 #pragma unroll
   for (int k1 = 0; k1 < A; ++k1) tile[(k1 * B + lg) * CBS + cl] = x[k1];
