@@ -654,6 +654,8 @@ pd_status pd_scan_create(const pd_scan_desc* desc, pd_scan** out) {
   }
   g.pl = 0;
   g.pp = g.n0;
+  g.il = 0;
+  g.ilsh = 0;
   h->N = (long long)g.n0 * g.n1 * g.n2;
   h->C = std::max(1, std::min(64, 4096 / g.n0));
   auto bad = [&](pd_status s, const std::string& m) {
@@ -856,6 +858,15 @@ pd_status pd_scan_set_ligand(pd_scan* h, int n_atoms, const double* pos, const d
   const int al = h->fcol ? h->fcol->lanes : 1;
   g.pl = g.lo[0];
   g.pp = std::min(g.n0, (g.p[0] + al - 1) / al * al);
+  // Interleaved L1 order for the fast column pass (whole-plane kernels only;
+  // the split path transforms L1 planes in place): needs whole 64/B column
+  // blocks and a group that its 32-bit offsets cover.
+  g.il = 0;
+  g.ilsh = 0;
+  if (h->fcol && !h->split && g.M % (64 / al) == 0 && (long long)g.pp * g.M * 8 <= 0xFFFFFFFFLL) {
+    g.il = al;
+    while ((1 << g.ilsh) < 64 / al) ++g.ilsh;
+  }
   if (!h->split && !plan_fits(h->hz, g.p[1], kPlaneThreads))
     return fail(PD_ERR_UNSUPPORTED, "ligand box too large for the register plan");
   hipFree(h->q); hipFree(h->w64); hipFree(h->w32);

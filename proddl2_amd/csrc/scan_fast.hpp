@@ -362,13 +362,23 @@ __global__ __launch_bounds__(256, PD_COL_WAVES) void k_col_fast(ScanGeom g, cons
   // what it consumes: a position m in [pp / B, MZ), or an iteration past the
   // last, reads 0 through rz.  Iteration it = (rotation bbeg + it / T, term
   // it % T); its planes start at it * pp * M complex in rl.
+  const bool il = g.il != 0;
+  // The wave's column block of the interleaved order.  That order is used only
+  // when M is a multiple of CW, so a wave's columns all exist or none does; a
+  // wave past the last column has nothing to do (no block barrier follows).
+  if (il && !colok) return;
+  const int colg = (m0 + wave * CW) / CW;
   auto load_planes = [&](int it, float2 (&x)[A]) {
-    const uint32_t st = (uint32_t)it * pM8 + (uint32_t)m0 * 8u;
+    const uint32_t st = (uint32_t)it * pM8;
 #pragma unroll
     for (int m = 0; m < A; ++m) {
       if (m < MZ) {
         const bool in = it < nit && B * m < pp;
-        x[m] = pd_ld2(in ? rl : rz, in ? (uint32_t)(lg + B * m) * M * 8u + vcol : 0u, in ? st : 0u);
+        // plane-major: plane lg + B m, column m0 + col; interleaved (g.il == B):
+      // one 512-byte block per (m, wave), read in lane order
+      const uint32_t o = il ? ((uint32_t)(m * (M / CW) + colg) * 64u + (uint32_t)lane) * 8u
+                            : (uint32_t)(lg + B * m) * M * 8u + vcol + (uint32_t)m0 * 8u;
+      x[m] = pd_ld2(in ? rl : rz, in ? o : 0u, in ? st : 0u);
       } else {
         x[m] = make_float2(0.f, 0.f);
       }
@@ -620,10 +630,18 @@ __global__ __launch_bounds__(512, PD_LIG_WAVES) void k_lig_plane_fwd_fast(ScanGe
         }
       }
       __syncthreads();
-      float2* dst = L1 + (((size_t)b * g.T + t) * g.pp + (g.lo[0] - g.pl) + p) * (size_t)(N1 * H2);
-      // plane stores through a buffer descriptor: 32-bit offsets instead of
-      // one 64-bit address per stored value
-      const __amdgpu_buffer_rsrc_t rd = pd_rsrc(dst, (uint32_t)(N1 * H2 * 8));
+      // stores through a buffer descriptor over the (rotation, term)'s planes:
+      // 32-bit offsets instead of one 64-bit address per stored value; in the
+      // interleaved order (g.il) column m of plane q sits at l1_off(g, q, m)
+      float2* dst = L1 + ((size_t)b * g.T + t) * g.pp * (size_t)(N1 * H2);
+      const int q = g.lo[0] - g.pl + p;
+      const __amdgpu_buffer_rsrc_t rd = pd_rsrc(dst, (uint32_t)g.pp * (N1 * H2 * 8));
+      const uint32_t qb = g.il ? (uint32_t)(((q / g.il) * ((N1 * H2) >> g.ilsh)) * 64 + (q % g.il) * (64 / g.il))
+                               : (uint32_t)q * (N1 * H2);
+      const uint32_t cmask = g.il ? (64u / g.il) - 1u : 0u;
+      auto off8 = [&](int m) -> uint32_t {  // byte offset of column m of this plane
+        return g.il ? (qb + ((uint32_t)(m >> g.ilsh) << 6) + ((uint32_t)m & cmask)) * 8u : (qb + (uint32_t)m) * 8u;
+      };
 #pragma unroll
       for (int r = 0; r < RY; ++r) {
         const int col = r * LY + cy;
@@ -637,8 +655,7 @@ __global__ __launch_bounds__(512, PD_LIG_WAVES) void k_lig_plane_fwd_fast(ScanGe
             rdft<B1, -1>(x);
             if (col != 0) {
 #pragma unroll
-              for (int k2 = 0; k2 < B1; ++k2)
-                pd_st2(rd, (uint32_t)(k1 * H2 + col) * 8u, (uint32_t)(A1 * k2 * H2 * 8), x[k2]);
+              for (int k2 = 0; k2 < B1; ++k2) pd_st2(rd, off8((k1 + A1 * k2) * H2 + col), 0u, x[k2]);
             } else {
               // packed spectrum C[k] parked in the unused slot column M2
 #pragma unroll
@@ -651,8 +668,8 @@ __global__ __launch_bounds__(512, PD_LIG_WAVES) void k_lig_plane_fwd_fast(ScanGe
       // Y0[k] = (C[k] + conj C[-k]) / 2, YM[k] = (C[k] - conj C[-k]) / (2i)
       for (int k = tid; k < N1; k += NT) {
         const float2 c = buf[k * LS + M2], cm = buf[((N1 - k) % N1) * LS + M2];
-        dst[k * H2] = make_float2(0.5f * (c.x + cm.x), 0.5f * (c.y - cm.y));
-        dst[k * H2 + M2] = make_float2(0.5f * (c.y + cm.y), -0.5f * (c.x - cm.x));
+        pd_st2(rd, off8(k * H2), 0u, make_float2(0.5f * (c.x + cm.x), 0.5f * (c.y - cm.y)));
+        pd_st2(rd, off8(k * H2 + M2), 0u, make_float2(0.5f * (c.y + cm.y), -0.5f * (c.x - cm.x)));
       }
       __syncthreads();
     }

@@ -41,10 +41,22 @@ struct ScanGeom {
   // grid shifted by pl along x: shifting both grids leaves the correlation
   // unchanged, and the ligand's nonzero positions start at 0.
   int pl, pp;
+  // L1 order within a (rotation, term): il = 0 plane after plane; il = B (the
+  // fast column pass's lanes per column, a power of two) in 512-byte blocks of
+  // B planes x 64/B columns, the order in which one wave of that pass reads a
+  // position of its transform.  ilsh = log2(64 / B).
+  int il, ilsh;
   double h;
   double zero_h[3];  // centred-geometry zero_ - h per axis
   double zsc[3];     // centred-geometry cell-0 centre per axis
 };
+
+// Offset (complex) of plane q, column m within one (rotation, term) of L1.
+__host__ __device__ __forceinline__ size_t l1_off(const ScanGeom& g, int q, int m) {
+  if (!g.il) return (size_t)q * g.M + m;
+  const int cw = 64 / g.il;
+  return ((size_t)(q / g.il) * (g.M >> g.ilsh) + (m >> g.ilsh)) * 64 + (q % g.il) * cw + (m & (cw - 1));
+}
 
 struct PeakOut {  // layout identical to pd_peak
   int32_t rot, rank, s[3], offset;
@@ -193,10 +205,11 @@ __global__ __launch_bounds__(512) void k_lig_plane_fwd(ScanGeom g, FftPlan pz, F
     fft_lds<-1>(buf + g.lo[1] * g.ls, pz, g.p[1], g.ls, 1);
     r2c_post(buf + g.lo[1] * g.ls, g.p[1], g.ls, g.m2, tw2);
     fft_lds<-1>(buf, py, g.h2, 1, g.ls);
-    float2* dst = L1 + (((size_t)b * g.T + t) * g.pp + (g.lo[0] - g.pl) + p) * (size_t)g.M;
+    float2* dst = L1 + ((size_t)b * g.T + t) * g.pp * (size_t)g.M;
+    const int q = g.lo[0] - g.pl + p;
     for (int e = threadIdx.x; e < g.M; e += blockDim.x) {
       const int k1 = e / g.h2, k2 = e - k1 * g.h2;
-      dst[e] = buf[k1 * g.ls + k2];
+      dst[l1_off(g, q, e)] = buf[k1 * g.ls + k2];
     }
     __syncthreads();
   }
@@ -221,11 +234,11 @@ __global__ __launch_bounds__(256) void k_col(ScanGeom g, FftPlan px, const float
 #pragma unroll
     for (int q = 0; q < PD_COL_PER; ++q) acc[q] = make_float2(0.f, 0.f);
     for (int t = 0; t < g.T; ++t) {
-      const float2* src = L1 + ((size_t)b * g.T + t) * g.pp * (size_t)g.M + m0;
+      const float2* src = L1 + ((size_t)b * g.T + t) * g.pp * (size_t)g.M;
       for (int e = threadIdx.x; e < nel; e += blockDim.x) {
         const int i0 = e / C, c = e - i0 * C;
         float2 v = make_float2(0.f, 0.f);
-        if (i0 < g.pp && c < cw) v = src[(size_t)i0 * g.M + c];
+        if (i0 < g.pp && c < cw) v = src[l1_off(g, i0, m0 + c)];
         buf[e] = v;
       }
       __syncthreads();

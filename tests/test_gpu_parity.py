@@ -58,6 +58,9 @@ CASES = [
     ((120, 120, 120), 1.2, 2000, 500, 2),
     # radix-7, -11 and -13 edges (findBestSize outputs such as 56, 44, 52; z half-length 26)
     ((56, 44, 52), 1.5, 300, 80, 2),
+    # edge-specialised column pass (N0 = 64) behind the generic plane kernels, with the
+    # ligand spectra in the column pass's interleaved order
+    ((64, 48, 40), 1.5, 300, 80, 4),
 ]
 
 
