@@ -343,9 +343,9 @@ __global__ __launch_bounds__(256, PD_COL_WAVES) void k_col_fast(ScanGeom g, cons
   // lg + B m is a stored plane for every lane (m < pp / B) or for none, and
   // positions m >= MZ >= pp / B are zero (not loaded).
   const int pp = g.pp;
-  // Lanes past the last column (last tile only) redo column 0 of the tile:
-  // the same loads and arithmetic give the same bits, so their stores are
-  // harmless duplicates.
+  // In the plane-major order, lanes past the last column (last tile only) redo
+  // column 0 of the tile: the same loads and arithmetic give the same bits, so
+  // their stores are harmless duplicates.
   const uint32_t vcol = colok ? (uint32_t)col * 8u : 0u;
   const uint32_t pM8 = (uint32_t)pp * M * 8u;
   // receptor spectra of all terms in lane order (k_rhat_lanes) and the
@@ -357,28 +357,29 @@ __global__ __launch_bounds__(256, PD_COL_WAVES) void k_col_fast(ScanGeom g, cons
   // zero-size buffer: every load through it is out of range and returns 0
   const __amdgpu_buffer_rsrc_t rz = pd_rsrc(L1, 0u);
   const int nit = (bend - bbeg) * T;
+  // Ligand order (g.il): plane-major, or blocks of B planes x CW columns in
+  // lane order.  The interleaved order is used only when M is a multiple of
+  // CW, so a wave's columns all exist or none does; a wave past the last
+  // column has nothing to do (no block barrier follows).
+  const bool il = g.il != 0;
+  if (il && !colok) return;
+  const int colg = (m0 + wave * CW) / CW;  // the wave's column block
   // Every memory instruction is issued unconditionally, so that the count of
   // outstanding loads is the same on every path and each wait covers only
   // what it consumes: a position m in [pp / B, MZ), or an iteration past the
   // last, reads 0 through rz.  Iteration it = (rotation bbeg + it / T, term
   // it % T); its planes start at it * pp * M complex in rl.
-  const bool il = g.il != 0;
-  // The wave's column block of the interleaved order.  That order is used only
-  // when M is a multiple of CW, so a wave's columns all exist or none does; a
-  // wave past the last column has nothing to do (no block barrier follows).
-  if (il && !colok) return;
-  const int colg = (m0 + wave * CW) / CW;
   auto load_planes = [&](int it, float2 (&x)[A]) {
     const uint32_t st = (uint32_t)it * pM8;
 #pragma unroll
     for (int m = 0; m < A; ++m) {
       if (m < MZ) {
         const bool in = it < nit && B * m < pp;
-        // plane-major: plane lg + B m, column m0 + col; interleaved (g.il == B):
-      // one 512-byte block per (m, wave), read in lane order
-      const uint32_t o = il ? ((uint32_t)(m * (M / CW) + colg) * 64u + (uint32_t)lane) * 8u
-                            : (uint32_t)(lg + B * m) * M * 8u + vcol + (uint32_t)m0 * 8u;
-      x[m] = pd_ld2(in ? rl : rz, in ? o : 0u, in ? st : 0u);
+        // plane-major: plane lg + B m, column m0 + col; interleaved: one
+        // 512-byte block per (m, wave), read in lane order
+        const uint32_t o = il ? ((uint32_t)(m * (M / CW) + colg) * 64u + (uint32_t)lane) * 8u
+                              : (uint32_t)(lg + B * m) * M * 8u + vcol + (uint32_t)m0 * 8u;
+        x[m] = pd_ld2(in ? rl : rz, in ? o : 0u, in ? st : 0u);
       } else {
         x[m] = make_float2(0.f, 0.f);
       }
