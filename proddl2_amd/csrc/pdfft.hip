@@ -450,7 +450,10 @@ pd_status launch_fft_path(pd_scan* h, const pd_scan::Lane& ln, int nb, const dou
     const int ntile = (g.M + C - 1) / C;
     int nyb = std::max(1, std::min(nb, 4096 / std::max(1, ntile)));
     if (fc) nyb = std::max(nyb, (int)((nb + max_bpb - 1) / max_bpb));
-    const int bpb = (nb + nyb - 1) / nyb;
+    int bpb = (nb + nyb - 1) / nyb;
+    // the fast column pass takes rotations in pairs: an even group size
+    // leaves no phantom rotation except at an odd batch end
+    if (fc && bpb % 2 && bpb < max_bpb) ++bpb;
     nyb = (nb + bpb - 1) / bpb;
     if (fc) {
       const int nblk = 8 * ((ntile + 7) / 8) * nyb;

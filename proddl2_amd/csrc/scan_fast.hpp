@@ -356,7 +356,6 @@ __global__ __launch_bounds__(256, PD_COL_WAVES) void k_col_fast(ScanGeom g, cons
   const __amdgpu_buffer_rsrc_t rs = pd_rsrc(S1 + (size_t)bbeg * g.n0 * M, (uint32_t)(bend - bbeg) * g.n0 * M * 8u);
   // zero-size buffer: every load through it is out of range and returns 0
   const __amdgpu_buffer_rsrc_t rz = pd_rsrc(L1, 0u);
-  const int nit = (bend - bbeg) * T;
   // Ligand order (g.il): plane-major, or blocks of B planes x CW columns in
   // lane order.  The interleaved order is used only when M is a multiple of
   // CW, so a wave's columns all exist or none does; a wave past the last
@@ -364,17 +363,24 @@ __global__ __launch_bounds__(256, PD_COL_WAVES) void k_col_fast(ScanGeom g, cons
   const bool il = g.il != 0;
   if (il && !colok) return;
   const int colg = (m0 + wave * CW) / CW;  // the wave's column block
+  // Rotations go in pairs, so that one load of a term's receptor values
+  // serves both.  Iteration j: pair j / 2T, term (j / 2) % T, member j % 2;
+  // an odd last rotation is paired with a phantom whose planes read 0 and
+  // whose stores fall outside rs (dropped by the range check).
+  const int nrel = bend - bbeg;
+  const int nj = ((nrel + 1) / 2) * 2 * T;
   // Every memory instruction is issued unconditionally, so that the count of
   // outstanding loads is the same on every path and each wait covers only
-  // what it consumes: a position m in [pp / B, MZ), or an iteration past the
-  // last, reads 0 through rz.  Iteration it = (rotation bbeg + it / T, term
-  // it % T); its planes start at it * pp * M complex in rl.
-  auto load_planes = [&](int it, float2 (&x)[A]) {
-    const uint32_t st = (uint32_t)it * pM8;
+  // what it consumes: a position m in [pp / B, MZ), a phantom rotation or an
+  // iteration past the last reads 0 through rz.  The planes of (rotation
+  // bbeg + rel, term t) start at (rel T + t) pp M complex in rl.
+  auto load_planes = [&](int j, float2 (&x)[A]) {
+    const int rel = 2 * (j / (2 * T)) + (j & 1), t = (j >> 1) % T;
+    const uint32_t st = (uint32_t)(rel * T + t) * pM8;
 #pragma unroll
     for (int m = 0; m < A; ++m) {
       if (m < MZ) {
-        const bool in = it < nit && B * m < pp;
+        const bool in = j < nj && rel < nrel && B * m < pp;
         // plane-major: plane lg + B m, column m0 + col; interleaved: one
         // 512-byte block per (m, wave), read in lane order
         const uint32_t o = il ? ((uint32_t)(m * (M / CW) + colg) * 64u + (uint32_t)lane) * 8u
@@ -385,30 +391,30 @@ __global__ __launch_bounds__(256, PD_COL_WAVES) void k_col_fast(ScanGeom g, cons
       }
     }
   };
-  float2 acc[J][B];
+  float2 acc0[J][B], acc1[J][B], rv[J][B];
 #pragma unroll
   for (int j = 0; j < J; ++j)
 #pragma unroll
-    for (int k = 0; k < B; ++k) acc[j][k] = make_float2(0.f, 0.f);
-  // One iteration: the term's receptor values, the next iteration's ligand
-  // planes into vn, then the forward transform of v and the accumulation;
-  // after the last term the inverse transform and the store.
-  auto step = [&](int it, float2 (&v)[A], float2 (&vn)[A]) {
-    const int t = it % T;
-    float2 rv[J][B];
-    const uint32_t rbase = ((uint32_t)(t * ntile + tile) * 4u + (uint32_t)wave) * NI * 1024u;
+    for (int k = 0; k < B; ++k) acc0[j][k] = acc1[j][k] = make_float2(0.f, 0.f);
+  // Forward transform of v (iteration j) into acc, with the next iteration's
+  // planes loaded into vn; member 0 first loads the term's receptor values.
+  auto fwd = [&](int j, float2 (&v)[A], float2 (&vn)[A], float2 (&acc)[J][B], bool load_rv) {
+    if (load_rv) {
+      const int t = (j >> 1) % T;
+      const uint32_t rbase = ((uint32_t)(t * ntile + tile) * 4u + (uint32_t)wave) * NI * 1024u;
 #pragma unroll
-    for (int i = 0; i < NI; ++i) {
-      const float4 q = pd_ld4(rr, (uint32_t)lane * 16u, rbase + (uint32_t)i * 1024u);
-      rv[(2 * i) / B][(2 * i) % B] = make_float2(q.x, q.y);
-      rv[(2 * i + 1) / B][(2 * i + 1) % B] = make_float2(q.z, q.w);
+      for (int i = 0; i < NI; ++i) {
+        const float4 q = pd_ld4(rr, (uint32_t)lane * 16u, rbase + (uint32_t)i * 1024u);
+        rv[(2 * i) / B][(2 * i) % B] = make_float2(q.x, q.y);
+        rv[(2 * i + 1) / B][(2 * i + 1) % B] = make_float2(q.z, q.w);
+      }
     }
     // Loads complete in issue order: the receptor values (L2 hits) go out
     // before the prefetch (HBM), so that the accumulation below does not
     // wait for the prefetch, and v's planes, older than both, are waited for
     // alone.  The barriers keep the scheduler from reordering the groups.
     __builtin_amdgcn_sched_barrier(0);
-    load_planes(it + 1, vn);
+    load_planes(j + 1, vn);
     __builtin_amdgcn_sched_barrier(0);
     four_step_phase1_zt<A, B, MZ, -1>(v, lg, twl);
 #pragma unroll
@@ -416,56 +422,63 @@ __global__ __launch_bounds__(256, PD_COL_WAVES) void k_col_fast(ScanGeom g, cons
     wave_lds_sync();
     float2 w[J][B];
 #pragma unroll
-    for (int j = 0; j < J; ++j) {
-      const int k1 = lg + B * j;
+    for (int jj = 0; jj < J; ++jj) {
+      const int k1 = lg + B * jj;
       if (k1 < A)
 #pragma unroll
-        for (int q = 0; q < B; ++q) w[j][q] = xb[k1 * XS + q * CW + cl];
+        for (int q = 0; q < B; ++q) w[jj][q] = xb[k1 * XS + q * CW + cl];
     }
     wave_lds_sync();
 #pragma unroll
-    for (int j = 0; j < J; ++j) {
-      const int k1 = lg + B * j;
+    for (int jj = 0; jj < J; ++jj) {
+      const int k1 = lg + B * jj;
       if (k1 < A) {
-        rdft<B, -1>(w[j]);
+        rdft<B, -1>(w[jj]);
 #pragma unroll
-        for (int k2 = 0; k2 < B; ++k2) acc[j][k2] = cadd(acc[j][k2], cmulc(w[j][k2], rv[j][k2]));
+        for (int k2 = 0; k2 < B; ++k2) acc[jj][k2] = cadd(acc[jj][k2], cmulc(w[jj][k2], rv[jj][k2]));
       }
     }
-    if (t == T - 1) {
-      // inverse along x: DFT_B over k2, twiddle, exchange, DFT_A over k1
-      const int b = it / T;
+  };
+  // Inverse along x of a finished rotation (DFT_B over k2, twiddle, exchange,
+  // DFT_A over k1), stored to S1, and the accumulator cleared.
+  const uint32_t sout = (uint32_t)nrel * g.n0 * M * 8u;  // first byte past rs
+  auto inv_store = [&](float2 (&acc)[J][B], int rel) {
 #pragma unroll
-      for (int j = 0; j < J; ++j) {
-        const int k1 = lg + B * j;
-        if (k1 < A) {
-          four_step_dif_phase1<A, B, +1>(acc[j], k1, twl);
+    for (int jj = 0; jj < J; ++jj) {
+      const int k1 = lg + B * jj;
+      if (k1 < A) {
+        four_step_dif_phase1<A, B, +1>(acc[jj], k1, twl);
 #pragma unroll
-          for (int q = 0; q < B; ++q) xb[k1 * XS + q * CW + cl] = acc[j][q];
-        }
+        for (int q = 0; q < B; ++q) xb[k1 * XS + q * CW + cl] = acc[jj][q];
       }
-      wave_lds_sync();
-      float2 x[A];
-#pragma unroll
-      for (int k1 = 0; k1 < A; ++k1) x[k1] = xb[k1 * XS + lg * CW + cl];
-      wave_lds_sync();
-      rdft<A, +1>(x);
-      const uint32_t sb = ((uint32_t)b * g.n0 * M + m0) * 8u;
-#pragma unroll
-      for (int m = 0; m < A; ++m) pd_st2(rs, (uint32_t)(lg + B * m) * M * 8u + vcol, sb, x[m]);
-#pragma unroll
-      for (int j = 0; j < J; ++j)
-#pragma unroll
-        for (int k = 0; k < B; ++k) acc[j][k] = make_float2(0.f, 0.f);
     }
+    wave_lds_sync();
+    float2 x[A];
+#pragma unroll
+    for (int k1 = 0; k1 < A; ++k1) x[k1] = xb[k1 * XS + lg * CW + cl];
+    wave_lds_sync();
+    rdft<A, +1>(x);
+    const bool ok = rel < nrel;
+    const uint32_t sb = ok ? ((uint32_t)rel * g.n0 * M + m0) * 8u : 0u;
+#pragma unroll
+    for (int m = 0; m < A; ++m) pd_st2(rs, ok ? (uint32_t)(lg + B * m) * M * 8u + vcol : sout, sb, x[m]);
+#pragma unroll
+    for (int jj = 0; jj < J; ++jj)
+#pragma unroll
+      for (int k = 0; k < B; ++k) acc[jj][k] = make_float2(0.f, 0.f);
   };
   // two plane buffers in turn, so no copy waits on the prefetch
   float2 va[A], vb[A];
   load_planes(0, va);
 #pragma unroll 1
-  for (int it = 0; it < nit; it += 2) {
-    step(it, va, vb);
-    if (it + 1 < nit) step(it + 1, vb, va);
+  for (int j = 0; j < nj; j += 2) {
+    fwd(j, va, vb, acc0, true);
+    fwd(j + 1, vb, va, acc1, false);
+    if ((j >> 1) % T == T - 1) {
+      const int rel0 = 2 * (j / (2 * T));
+      inv_store(acc0, rel0);
+      inv_store(acc1, rel0 + 1);
+    }
   }
 }
 
