@@ -13,6 +13,12 @@
 // Sign convention is FFTW's: DIR = -1 forward e^{-2 pi i jk/n}, DIR = +1
 // backward; no normalisation.
 #pragma once
+// FP contraction only within one source expression (a * b + c written as
+// such), never chosen by the optimiser across statements: the column pass
+// inlines the same transform twice (one copy per rotation of a pair), and
+// optimiser-chosen fusion could round the two copies differently, making a
+// rotation's result depend on its place in the batch.
+#pragma clang fp contract(on)
 #include <hip/hip_runtime.h>
 
 #ifndef PD_MAXV

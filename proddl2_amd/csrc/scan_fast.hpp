@@ -485,6 +485,7 @@ __global__ __launch_bounds__(256, PD_COL_WAVES) void k_col_fast(ScanGeom g, cons
 // ---------------------------------------------------------------------------
 // Ligand plane pass.  grid (p0, nrot), block 512, LDS N1 * LS complex.
 constexpr int kLigList = 1024;  // atoms of one plane kept in the block's list
+constexpr int kLigTerms = 4;    // terms whose weights the list keeps in LDS
 // ---------------------------------------------------------------------------
 template <int N1, int N2>
 __global__ __launch_bounds__(512, PD_LIG_WAVES) void k_lig_plane_fwd_fast(ScanGeom g, const float2* __restrict__ twN1,
@@ -515,12 +516,15 @@ __global__ __launch_bounds__(512, PD_LIG_WAVES) void k_lig_plane_fwd_fast(ScanGe
   // slower: they share the LDS wait counter.)
   static_assert(LZ % 64 == 0, "one lane group per wave");
   const int gzu = __builtin_amdgcn_readfirstlane(gz);
-  // The plane's atoms, found once for all terms: atom index and position in
-  // the plane buffer (as a float index).  More than kLigList atoms in one
-  // plane fall back to scanning every atom per term.
+  // The plane's atoms, found once for all terms: atom index, position in the
+  // plane buffer (as a float index) and, for up to kLigTerms terms, the
+  // weights, so that the deposits of later terms read LDS only.  More than
+  // kLigList atoms in one plane fall back to scanning every atom per term.
   __shared__ int s_cnt;
   __shared__ int s_atom[kLigList];
   __shared__ int s_pos[kLigList];
+  __shared__ float s_w[kLigTerms][kLigList];
+  const bool wl = g.T <= kLigTerms;
   if (tid == 0) s_cnt = 0;
   __syncthreads();
 #pragma unroll 4
@@ -533,6 +537,8 @@ __global__ __launch_bounds__(512, PD_LIG_WAVES) void k_lig_plane_fwd_fast(ScanGe
       if (k < kLigList) {
         s_atom[k] = j;
         s_pos[k] = i1 * 2 * LS + i2;
+        if (wl)
+          for (int t = 0; t < g.T; ++t) s_w[t][k] = w[(size_t)t * n_atoms + j];
       }
     }
   }
@@ -542,7 +548,9 @@ __global__ __launch_bounds__(512, PD_LIG_WAVES) void k_lig_plane_fwd_fast(ScanGe
     for (int e = tid; e < p1 * M2; e += NT) buf[(lo1 + e / M2) * LS + e % M2] = make_float2(0.f, 0.f);
     __syncthreads();
     const float* wt = w + (size_t)t * n_atoms;
-    if (cnt <= kLigList) {
+    if (cnt <= kLigList && wl) {
+      for (int k = tid; k < cnt; k += NT) atomicAdd(&fb[s_pos[k]], s_w[t][k]);
+    } else if (cnt <= kLigList) {
       for (int k = tid; k < cnt; k += NT) atomicAdd(&fb[s_pos[k]], wt[s_atom[k]]);
     } else {
       for (int j = tid; j < n_atoms; j += NT) {
