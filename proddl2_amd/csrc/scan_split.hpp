@@ -117,11 +117,21 @@ __global__ __launch_bounds__(256) void k_cols_fast(ScanGeom g, const float2* __r
     tw[k1] = twN1[lg * k1];
     if (DIR > 0) tw[k1].y = -tw[k1].y;
   }
-  for (int e = threadIdx.x; e < N1 * CB; e += 256) {
+  // N1 * CB / 256 elements per thread, all loads in flight before the first
+  // LDS write (a rolled loop would wait for each load in turn)
+  static_assert(N1 * CB % 256 == 0, "whole tile rows per pass");
+  constexpr int NE = N1 * CB / 256;
+  float2 ld[NE];
+#pragma unroll
+  for (int i = 0; i < NE; ++i) {
+    const int e = threadIdx.x + 256 * i;
     const int y = e / CB, c = e % CB;
-    float2 v = make_float2(0.f, 0.f);
-    if (c < cw && y >= ylo && y < yhi) v = pl[(size_t)y * g.h2 + c];
-    tile[y * CBS + c] = v;
+    ld[i] = (c < cw && y >= ylo && y < yhi) ? pl[(size_t)y * g.h2 + c] : make_float2(0.f, 0.f);
+  }
+#pragma unroll
+  for (int i = 0; i < NE; ++i) {
+    const int e = threadIdx.x + 256 * i;
+    tile[(e / CB) * CBS + e % CB] = ld[i];
   }
   __syncthreads();
   float2 x[A];
@@ -153,7 +163,9 @@ __global__ __launch_bounds__(256) void k_cols_fast(ScanGeom g, const float2* __r
     }
   }
   __syncthreads();
-  for (int e = threadIdx.x; e < N1 * CB; e += 256) {
+#pragma unroll
+  for (int i = 0; i < NE; ++i) {
+    const int e = threadIdx.x + 256 * i;
     const int y = e / CB, c = e % CB;
     if (c < cw) pl[(size_t)y * g.h2 + c] = tile[y * CBS + c];
   }
