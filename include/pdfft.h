@@ -71,6 +71,10 @@ typedef struct pd_scan_desc {
 /* pd_scan_desc.flags: run all batches on the handle's stream (no overlap of
  * consecutive batches), e.g. to profile each kernel alone. */
 #define PD_SCAN_ONE_LANE 2
+/* pd_scan_desc.flags: with the row and column kernels, run the inverse y pass
+ * and the inverse rows as two kernels also where one kernel holds the whole
+ * x plane on chip (N1 = N2 = 256), e.g. to compare the two. */
+#define PD_SCAN_SPLIT_INVERSE 4
 
 typedef struct pd_term {
   int kind;               /* PD_FIELD_*                                                      */

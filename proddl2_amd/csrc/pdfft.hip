@@ -111,6 +111,14 @@ const FastYCols* find_fast_ycols(int n1) {
   return nullptr;
 }
 
+// Fused inverse y pass + rows + top-K of the split path (N1 = N2 = 256).
+using InvSplitFn = void (*)(ScanGeom, const float2*, const float2*, const float2*, const float2*, float, int,
+                           float*, int*, float*, unsigned*);
+constexpr int kInvSplitN = 256;
+constexpr int kInvSplitLds = 128 * (kInvSplitN / 2 + 1) * 8;  // 128 rows of the half plane
+static_assert(kInvSplitLds >= 16 * 16 * 64 * 8, "y exchange fits the row buffer");
+const InvSplitFn kInvSplit = k_inv_plane_topk_split<kInvSplitN, kInvSplitN>;
+
 const FastRows* find_fast_rows(int n2) {
   for (const FastRows& f : kFastRows)
     if (f.n2 == n2) return &f;
@@ -228,6 +236,7 @@ struct pd_scan {
   const FastCol* fcol = nullptr;  // column kernel specialised on N0, if any
   const FastRows* frows = nullptr;  // split inverse rows specialised on N2, if any
   const FastYCols* fycol = nullptr;  // split y passes specialised on N1, if any
+  bool inv_fused = false;            // split inverse as one kernel (N1 = N2 = 256)
   bool split = false;  // plane passes as row + column kernels (scan_split.hpp)
   HostPlan hx, hy, hz;
   FftPlan px{}, py{}, pz{};
@@ -468,7 +477,10 @@ pd_status launch_fft_path(pd_scan* h, const pd_scan::Lane& ln, int nb, const dou
   }
   {
     Timer tm(h, 3, stream);
-    if (h->split) {
+    if (h->split && h->inv_fused) {
+      hipLaunchKernelGGL(kInvSplit, dim3(nb, g.n0), dim3(1024), kInvSplitLds, stream, g, h->tw_y, h->tw_z, h->tw2,
+                         ln.S1, scale, h->K, ln.cval, ln.coff, grid_out, ln.thr);
+    } else if (h->split) {
       if (h->fycol)
         hipLaunchKernelGGL(h->fycol->inv, dim3(nb * g.n0, (g.h2 + h->fycol->cols - 1) / h->fycol->cols), dim3(256),
                            h->fycol->lds, stream, g, h->tw_y, ln.S1, g.n0, g.n0, 0, 0, g.n1);
@@ -730,6 +742,9 @@ pd_status pd_scan_create(const pd_scan_desc* desc, pd_scan** out) {
     h->fycol = find_fast_ycols(g.n1);
     if (h->fycol && ((st = set_lds((const void*)h->fycol->fwd, h->fycol->lds)) != PD_OK ||
                      (st = set_lds((const void*)h->fycol->inv, h->fycol->lds)) != PD_OK))
+      return bad(st, pd_detail::last_error());
+    h->inv_fused = g.n1 == kInvSplitN && g.n2 == kInvSplitN && !(h->d.flags & PD_SCAN_SPLIT_INVERSE);
+    if (h->inv_fused && (st = set_lds((const void*)kInvSplit, kInvSplitLds)) != PD_OK)
       return bad(st, pd_detail::last_error());
   }
   if (h->fast) {

@@ -459,3 +459,261 @@ __global__ __launch_bounds__(512) void k_inv_rows_topk_fast(ScanGeom g, const fl
   }
   if (thr && tid == 0 && best[K - 1] != kNoKey) atomicMin(thr + b, (unsigned)(best[K - 1] >> 32));
 }
+
+// ---------------------------------------------------------------------------
+// Inverse y pass, inverse rows and top-K in one kernel, for N1 = 256 and
+// N2 = 256: the x plane of S1 is read from HBM once and stays on chip.
+// grid (nrot, n0), block 1024 = 16 waves; lane c and wave r.
+//   1. Loads: for the column sets s (kz = c + 64 s, s < M2 / 64), the lane
+//      holds rows r + 16 m (m < 16); each wave load reads 512 contiguous
+//      bytes of one row.  Column kz = M2 (one more than the lanes hold) is
+//      transformed by lanes 0..15 of wave 0 alone, with the same four-step
+//      (exchange through a 2 KiB LDS array); its real part goes to LDS.
+//   2. Backward y FFT, four-step 16 x 16: DFT over m, twiddle W_256^{r k1}
+//      (wave-uniform), one exchange of the set through LDS (128 KiB), DFT
+//      over the 16 waves.  The lane then holds rows r + 16 k2.
+//   3. Per half of the rows (128 rows x M2 + 1 complex, 129 KiB of LDS): the
+//      lanes write their rows of the half, then the rows take the c2r
+//      pre-step, the half-length backward four-step FFT (LZ = 128 rows of 8
+//      lanes, exchange in place), the scale, and the K-best selection under
+//      the rotation's threshold with a block-wide running list, as in
+//      k_inv_rows_topk_fast.
+// Same arithmetic, in the same order, as k_cols_fast<N1, +1> followed by
+// k_inv_rows_topk_fast<N2>, so the correlation values are the same bits.
+// LDS: max(16 * 16 * 64, 128 * (M2 + 1)) complex + lists.
+// ---------------------------------------------------------------------------
+template <int N1, int N2>
+__global__ __launch_bounds__(1024) void k_inv_plane_topk_split(ScanGeom g, const float2* __restrict__ twN1,
+                                                               const float2* __restrict__ twM2,
+                                                               const float2* __restrict__ tw2,
+                                                               const float2* __restrict__ S1, float scale, int K,
+                                                               float* __restrict__ cval, int* __restrict__ coff,
+                                                               float* __restrict__ grid_out,
+                                                               unsigned* __restrict__ thr) {
+  constexpr int NT = 1024, NW = NT / 64, L = 4;
+  constexpr int A1 = 16, B1 = NW;  // y four-step: 16 values per lane x 16 waves
+  static_assert(A1 * B1 == N1, "y split");
+  constexpr int M2 = N2 / 2, H2 = M2 + 1, LS = H2 | 1, NS = M2 / 64;
+  static_assert(M2 % 64 == 0, "whole column sets");
+  using SZ = Split<M2>;
+  constexpr int A2 = SZ::A, B2 = SZ::B, J2 = SZ::J;
+  constexpr int LZ = NT / B2, NH = N1 / LZ;
+  static_assert(N1 % LZ == 0 && LZ % 64 == 0 && A1 % NH == 0, "halves of whole rows");
+  constexpr int KH = A1 / NH;  // k2 values per half
+  extern __shared__ float2 buf[];
+  __shared__ uint64_t wtop[NW][32];
+  __shared__ uint64_t best[32];
+  __shared__ uint32_t s_thr;
+  __shared__ int s_any[NH];
+  __shared__ float2 s_nyq[N1];  // exchange of the kz = M2 column's FFT
+  __shared__ float s_g[N1];     // that column after the y FFT (real part)
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int b = blockIdx.x, i0 = blockIdx.y;
+  const float2* src = S1 + ((size_t)b * g.n0 + i0) * (size_t)(N1 * H2);
+  const float INF = __int_as_float(0x7f800000);
+  const int plane = N1 * N2, base = i0 * plane;
+  float2 x[NS][A1];
+#pragma unroll
+  for (int s = 0; s < NS; ++s)
+#pragma unroll
+    for (int m = 0; m < A1; ++m) x[s][m] = src[(wave + B1 * m) * H2 + lane + 64 * s];
+  if (wave == 0 && lane < B1) {
+    // column kz = M2: the same four-step on lanes 0..15 of one wave, lane r
+    // holding rows r + 16 m; the exchange stays within the wave
+    float2 v[A1];
+#pragma unroll
+    for (int m = 0; m < A1; ++m) v[m] = src[(lane + B1 * m) * H2 + M2];
+    rdft<A1, +1>(v);
+#pragma unroll
+    for (int k1 = 1; k1 < A1; ++k1) {
+      float2 w = twN1[lane * k1];
+      w.y = -w.y;
+      v[k1] = cmul(v[k1], w);
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < A1; ++k1) s_nyq[k1 * B1 + lane] = v[k1];
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int q = 0; q < B1; ++q) v[q] = s_nyq[lane * B1 + q];
+    rdft<B1, +1>(v);
+#pragma unroll
+    for (int k2 = 0; k2 < B1; ++k2) s_g[lane + A1 * k2] = v[k2].x;
+  }
+  if (tid < 32) best[tid] = kNoKey;
+  if (tid < NH) s_any[tid] = 0;
+  if (tid == 0)
+    s_thr = thr ? __hip_atomic_load(thr + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0xFFFFFFFFu;
+  // backward y FFT of each set
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    rdft<A1, +1>(x[s]);
+#pragma unroll
+    for (int k1 = 1; k1 < A1; ++k1) {
+      float2 w = twN1[wave * k1];
+      w.y = -w.y;
+      x[s][k1] = cmul(x[s][k1], w);
+    }
+    if (s > 0) __syncthreads();  // the previous set's reads are done
+#pragma unroll
+    for (int k1 = 0; k1 < A1; ++k1) buf[(k1 * B1 + wave) * 64 + lane] = x[s][k1];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < B1; ++q) x[s][q] = buf[(wave * B1 + q) * 64 + lane];
+    rdft<B1, +1>(x[s]);  // x[s][k2] is row wave + 16 k2
+    // finish the DFT here: held as final values the rows of the second half
+    // take 2 registers each, as shared butterfly operands they take more
+#pragma unroll
+    for (int k = 0; k < A1; ++k) asm volatile("" : "+v"(x[s][k].x), "+v"(x[s][k].y));
+  }
+  const int gz = tid / LZ, rz = tid % LZ;
+  const int gzu = __builtin_amdgcn_readfirstlane(gz);
+#pragma unroll
+  for (int hh = 0; hh < NH; ++hh) {
+    __syncthreads();  // LDS reads of the previous phase are done
+#pragma unroll
+    for (int s = 0; s < NS; ++s)
+#pragma unroll
+      for (int k = 0; k < KH; ++k) {
+        const int yl = wave + B1 * k;
+        const float2 v = x[s][KH * hh + k];
+        buf[yl * LS + lane + 64 * s] = v;
+        if (s == 0 && lane == 0) buf[yl * LS + M2] = make_float2(s_g[LZ * hh + yl], 0.f);
+      }
+    __syncthreads();
+    const int row = LZ * hh + rz;
+    // c2r pre-step in two groups, so that only half of its loads are in
+    // flight next to the rows held for the second half
+    float2 z[A2];
+#pragma unroll
+    for (int m = 0; m < A2; ++m) {
+      if (m == A2 / 2) __builtin_amdgcn_sched_barrier(0);
+      const int k = gz + B2 * m;
+      float2 xk = buf[rz * LS + k], xm = buf[rz * LS + M2 - k];
+      if (k == 0) { xk.y = 0.f; xm.y = 0.f; }
+      z[m] = c2r_point(xk, xm, tw2[gzu + B2 * m]);
+    }
+    four_step_phase1<A2, B2, +1>(z, gzu, twM2);
+    __syncthreads();  // every row of the half is read before the exchange
+#pragma unroll
+    for (int k1 = 0; k1 < A2; ++k1) buf[rz * LS + k1 * B2 + gz] = z[k1];
+    __syncthreads();
+    // second DFT of each k1; the results go back to the slots the lane read
+    // (no other lane reads them), and the selection reads them from there,
+    // which keeps the registers for the rows of the next half
+    float lmin = INF;
+#pragma unroll
+    for (int j = 0; j < J2; ++j) {
+      const int k1 = gz + B2 * j;
+      if (j > 0) __builtin_amdgcn_sched_barrier(0);
+      if (k1 < A2) {
+        float2 u[B2];
+#pragma unroll
+        for (int q = 0; q < B2; ++q) u[q] = buf[rz * LS + k1 * B2 + q];
+        rdft<B2, +1>(u);
+#pragma unroll
+        for (int k2 = 0; k2 < B2; ++k2) {
+          const int e = row * N2 + 2 * (k1 + A2 * k2);
+          float vx = u[k2].x * scale, vy = u[k2].y * scale;
+          if (grid_out) {
+            float* go = grid_out + ((size_t)b * g.n0 + i0) * (size_t)plane;
+            go[e] = vx;
+            go[e + 1] = vy;
+          }
+          if (vx != vx) vx = INF;
+          if (vy != vy) vy = INF;
+          buf[rz * LS + k1 * B2 + k2] = make_float2(vx, vy);
+          lmin = fminf(lmin, fminf(vx, vy));
+        }
+      }
+    }
+    asm volatile("" ::: "memory");  // read the values back from LDS below
+    const uint64_t bk = best[K - 1];
+    float tval = s_thr == 0xFFFFFFFFu ? INF : pd_key_value((uint64_t)s_thr << 32);
+    if (bk != kNoKey) tval = fminf(tval, pd_key_value(bk));
+    if (__ballot(lmin <= tval) == 0ull) {
+      if (lane < K) wtop[wave][lane] = kNoKey;
+    } else {
+      if (lane == 0) s_any[hh] = 1;
+      const float tau = fminf(wave_kth_smallest(lmin, K), tval);
+      uint64_t h[L];
+#pragma unroll
+      for (int i = 0; i < L; ++i) h[i] = kNoKey;
+#pragma unroll
+      for (int j = 0; j < J2; ++j) {
+        const int k1 = gz + B2 * j;
+        if (k1 < A2) {
+#pragma unroll
+          for (int k2 = 0; k2 < B2; ++k2) {
+            const int e = base + row * N2 + 2 * (k1 + A2 * k2);
+            const float2 v = buf[rz * LS + k1 * B2 + k2];
+            if (v.x <= tau) list_insert(h, pd_key(v.x, e));
+            if (v.y <= tau) list_insert(h, pd_key(v.y, e + 1));
+          }
+        }
+      }
+      uint64_t last = 0;
+      int popped = 0;
+      for (int rk = 0; rk < K; ++rk) {
+        const uint64_t m = wave_min_u64(h[0]);
+        if (lane == 0) wtop[wave][rk] = m;
+        if (m == kNoKey) {
+          if (lane > rk && lane < K) wtop[wave][lane] = kNoKey;
+          break;
+        }
+        if (h[0] == m) {
+          last = m;
+          list_pop(h);
+          ++popped;
+        }
+        if (h[0] == kNoKey && popped > 0 && popped % L == 0) {
+#pragma unroll
+          for (int j = 0; j < J2; ++j) {
+            const int k1 = gz + B2 * j;
+            if (k1 < A2) {
+#pragma unroll
+              for (int k2 = 0; k2 < B2; ++k2) {
+                const int e = base + row * N2 + 2 * (k1 + A2 * k2);
+                const float2 v = buf[rz * LS + k1 * B2 + k2];
+                const uint64_t kx = pd_key(v.x, e), ky = pd_key(v.y, e + 1);
+                if (kx > last) list_insert(h, kx);
+                if (ky > last) list_insert(h, ky);
+              }
+            }
+          }
+        }
+      }
+    }
+    __syncthreads();
+    // wave 0 merges the waves' candidates with the running list, when any
+    // wave had one under the threshold
+    if (wave == 0 && s_any[hh]) {
+      constexpr int NQ = (NW * 32 + 32 + 63) / 64;
+      uint64_t hq[NQ];
+#pragma unroll
+      for (int i = 0; i < NQ; ++i) hq[i] = kNoKey;
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        const int idx = lane + 64 * q;
+        if (idx < NW * K)
+          list_insert(hq, wtop[idx / K][idx % K]);
+        else if (idx < NW * K + K)
+          list_insert(hq, best[idx - NW * K]);
+      }
+      for (int rk = 0; rk < K; ++rk) {
+        const uint64_t m = wave_min_u64(hq[0]);
+        if (m == kNoKey) break;  // best[rk..] hold no key already
+        if (lane == 0) best[rk] = m;
+        if (hq[0] == m) list_pop(hq);
+      }
+    }
+  }
+  __syncthreads();
+  if (tid < K) {
+    const uint64_t m = best[tid];
+    const size_t at = ((size_t)b * g.n0 + i0) * K + tid;
+    cval[at] = m == kNoKey ? INF : pd_key_value(m);
+    coff[at] = m == kNoKey ? -1 : (int)(uint32_t)m;
+  }
+  if (thr && tid == 0 && best[K - 1] != kNoKey) atomicMin(thr + b, (unsigned)(best[K - 1] >> 32));
+}

@@ -65,6 +65,7 @@ class ScanDesc(C.Structure):
 
 SCAN_SPLIT_PLANES = 1
 SCAN_ONE_LANE = 2
+SCAN_SPLIT_INVERSE = 4
 
 
 class Term(C.Structure):

@@ -2,8 +2,8 @@
 
 Bars:
   * receptor projection (FP64 on the GPU): bit-identical to the oracle;
-  * correlation values (FP32 pipeline): |gpu - oracle| <= tol, with
-    tol = TOL_REL * sum_t ||w_t||_2 * ||R_t||_2 / sqrt(N0 N1 N2) * sqrt(N0 N1 N2)/... (see tolerance());
+  * correlation values (FP32 pipeline): |gpu - oracle| <= tol (see tolerance()); for
+    whole grids the larger of tol and 4 eps32 max|C| (see grid_tolerance());
   * selected translations (integers): identical to the oracle's at every rank
     whose oracle value is separated from its neighbours by more than 2*tol;
   * FP64 rescoring: within 1e-9 relative of the oracle's direct sum.
@@ -32,6 +32,15 @@ def tolerance(rec_grids, w, n):
     N = int(np.prod(n))
     s = sum(np.linalg.norm(w[t]) * np.linalg.norm(rec_grids[t]) for t in range(len(w)))
     return 32.0 * EPS32 * np.sqrt(np.log2(N)) * s / np.sqrt(N)
+
+
+def grid_tolerance(tol, ref):
+    """Bar for a whole correlation grid: the norm-based tol, or 4 eps32 of the
+    largest |value|, whichever is larger.  The last passes round each value
+    relative to its own magnitude; on grids much larger than the molecules the
+    values are sparse, so ||L|| ||R|| / sqrt(N) underestimates the largest ones
+    (the cases here show max errors of 1.1 to 3.1 eps32 max|C|)."""
+    return max(tol, 4.0 * EPS32 * float(np.abs(ref).max()))
 
 
 def build_case(oracle, n, h, n_rec, n_lig, n_terms, seed):
@@ -84,6 +93,10 @@ CASES_BOTH = [c + (0,) for c in CASES] + [
     ((96, 96, 96), 1.2, 1500, 300, 4, SPLIT),
     # z half-length 26 has no register-FFT split: the runtime-plan row kernel
     ((56, 44, 52), 1.5, 300, 80, 2, SPLIT),
+    # N1 = N2 = 256 (a 256 x 129 plane exceeds LDS): the inverse y pass, rows and
+    # selection in one kernel, and the same case through the two inverse kernels
+    ((32, 256, 256), 1.5, 300, 80, 2, 0),
+    ((32, 256, 256), 1.5, 300, 80, 2, pdfft.SCAN_SPLIT_INVERSE),
 ]
 
 
@@ -101,8 +114,10 @@ def test_correlation_grid(oracle, n, h, n_rec, n_lig, T, flags):
             ref = oracle.correlate(S, L)
             got = sc.correlation_grid(r).astype(np.float64)
             err = np.abs(got - ref).max()
-            print(f"  max|err|={err:.3e} tol={tol:.3e} range=[{ref.min():.3e},{ref.max():.3e}]", flush=True)
-            assert err <= tol
+            bar = grid_tolerance(tol, ref)
+            print(f"  max|err|={err:.3e} tol={tol:.3e} bar={bar:.3e} range=[{ref.min():.3e},{ref.max():.3e}]",
+                  flush=True)
+            assert err <= bar
 
 
 # Ligand boxes of different widths select different column kernel variants:
@@ -192,6 +207,27 @@ def check_peaks(oracle, n, h, n_rec, n_lig, T, flags, n_rot, seed, K=10):
     assert np.abs(e64 - peaks["value"].reshape(-1)).max() <= 3 * tol
     print(f"  checked {checked} of {len(rots) * K} ranks exactly (tol={tol:.3e})", flush=True)
     assert checked >= len(rots) * K // 2
+
+
+def test_one_kernel_inverse_same_bits_as_two(oracle):
+    """N1 = N2 = 256: the one-kernel inverse (y pass, rows and selection with the
+    plane on chip) does the arithmetic of the two inverse kernels in the same
+    order: correlation grids and peaks are bit-identical."""
+    n, h, T, K = (32, 256, 256), 1.5, 2, 10
+    print(f"case one-kernel vs two-kernel inverse n={n}", flush=True)
+    cx, rt, w, R = build_case(oracle, n, h, 300, 80, T, seed=14)
+    rots = synth.rotation_set(9, seed=5)
+    out = []
+    for flags in (0, pdfft.SCAN_SPLIT_INVERSE):
+        with make_scan(n, h, T, K, cx, rt, w, batch=4, flags=flags) as sc:
+            grids = [sc.correlation_grid(r) for r in rots[:2]]
+            peaks, nout = sc.run(rots)
+        out.append((grids, peaks, nout))
+    for a, b in zip(out[0][0], out[1][0]):
+        assert np.array_equal(a, b), np.abs(a - b).max()
+    assert np.array_equal(out[0][2], out[1][2])
+    for f in ("rot", "rank", "s", "offset", "value"):  # value64 is NaN (not rescored)
+        assert np.array_equal(out[0][1][f], out[1][1][f]), f
 
 
 def test_stress_256(oracle):
