@@ -75,6 +75,9 @@ typedef struct pd_scan_desc {
  * and the inverse rows as two kernels also where one kernel holds the whole
  * x plane on chip (N1 = N2 = 256), e.g. to compare the two. */
 #define PD_SCAN_SPLIT_INVERSE 4
+/* pd_scan_desc.flags: likewise for the ligand plane pass (deposit, rows along
+ * z and the y pass as separate kernels where one kernel would do them). */
+#define PD_SCAN_SPLIT_FORWARD 8
 
 typedef struct pd_term {
   int kind;               /* PD_FIELD_*                                                      */

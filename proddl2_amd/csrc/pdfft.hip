@@ -118,6 +118,11 @@ constexpr int kInvSplitN = 256;
 constexpr int kInvSplitLds = 128 * (kInvSplitN / 2 + 1) * 8;  // 128 rows of the half plane
 static_assert(kInvSplitLds >= 16 * 16 * 64 * 8, "y exchange fits the row buffer");
 const InvSplitFn kInvSplit = k_inv_plane_topk_split<kInvSplitN, kInvSplitN>;
+// ... and the one-kernel ligand plane pass (box rows up to kPlaneSplitRows).
+using LigSplitFn = void (*)(ScanGeom, const float2*, const float2*, const float2*, const int*, int, const float*,
+                            float2*);
+const LigSplitFn kLigSplit = k_lig_plane_split<kInvSplitN, kInvSplitN>;
+static_assert(kInvSplitLds == kPlaneSplitRows * (kInvSplitN / 2 + 1) * 8, "box rows fill the row buffer");
 
 const FastRows* find_fast_rows(int n2) {
   for (const FastRows& f : kFastRows)
@@ -237,6 +242,7 @@ struct pd_scan {
   const FastRows* frows = nullptr;  // split inverse rows specialised on N2, if any
   const FastYCols* fycol = nullptr;  // split y passes specialised on N1, if any
   bool inv_fused = false;            // split inverse as one kernel (N1 = N2 = 256)
+  bool lig_fused = false;            // split ligand pass as one kernel (N1 = N2 = 256)
   bool split = false;  // plane passes as row + column kernels (scan_split.hpp)
   HostPlan hx, hy, hz;
   FftPlan px{}, py{}, pz{};
@@ -428,7 +434,10 @@ pd_status launch_fft_path(pd_scan* h, const pd_scan::Lane& ln, int nb, const dou
   const FastSet* f = h->fast;
   {
     Timer tm(h, 1, stream);
-    if (h->split) {
+    if (h->split && h->lig_fused && g.p[1] <= kPlaneSplitRows && g.il == 0) {
+      hipLaunchKernelGGL(kLigSplit, dim3(g.p[0], nb, g.T), dim3(1024), kInvSplitLds, stream, g, h->tw_y, h->tw_z, h->tw2,
+                         ln.cells, h->n_lig, h->w32, ln.L1);
+    } else if (h->split) {
       hipLaunchKernelGGL(k_lig_rows, dim3(g.p[0], nb, (g.p[1] + kSplitRows - 1) / kSplitRows),
                          dim3(kSplitThreads), split_row_lds(h), stream, g, h->pz, h->tw2, ln.cells, h->n_lig,
                          h->w32, ln.L1);
@@ -745,6 +754,9 @@ pd_status pd_scan_create(const pd_scan_desc* desc, pd_scan** out) {
       return bad(st, pd_detail::last_error());
     h->inv_fused = g.n1 == kInvSplitN && g.n2 == kInvSplitN && !(h->d.flags & PD_SCAN_SPLIT_INVERSE);
     if (h->inv_fused && (st = set_lds((const void*)kInvSplit, kInvSplitLds)) != PD_OK)
+      return bad(st, pd_detail::last_error());
+    h->lig_fused = g.n1 == kInvSplitN && g.n2 == kInvSplitN && !(h->d.flags & PD_SCAN_SPLIT_FORWARD);
+    if (h->lig_fused && (st = set_lds((const void*)kLigSplit, kInvSplitLds)) != PD_OK)
       return bad(st, pd_detail::last_error());
   }
   if (h->fast) {

@@ -717,3 +717,158 @@ __global__ __launch_bounds__(1024) void k_inv_plane_topk_split(ScanGeom g, const
   }
   if (thr && tid == 0 && best[K - 1] != kNoKey) atomicMin(thr + b, (unsigned)(best[K - 1] >> 32));
 }
+
+// ---------------------------------------------------------------------------
+// Ligand plane pass in one kernel for N1 = 256 and N2 = 256 with at most 128
+// box rows: the plane's spectrum is written once, as whole rows.
+// grid (p0, nrot, T), block 1024 = 16 waves; one term per block:
+//   1. deposit the plane's atoms into the box rows (real rows in LDS, row
+//      stride LS complex);
+//   2. packed half-length forward z FFT of each box row (register four-step,
+//      one row per 8 lanes, exchange in place);
+//   3. r2c post-step in place (X[k] and X[M2 - k] from Z[k] and Z[M2 - k]);
+//   4. the lanes load their column sets (lane c, wave r: kz = c + 64 s, rows
+//      r + 16 m, zero outside the box), forward y FFT four-step 16 x 16 with
+//      one LDS exchange per set, stores of 512 contiguous bytes per wave.
+//      Column kz = M2 is transformed by lanes 0..15 of wave 0 alone.
+// LDS: 128 * LS complex (the box rows, then the y exchange).
+// ---------------------------------------------------------------------------
+constexpr int kPlaneSplitRows = 128;  // box rows the one-kernel ligand pass holds
+
+template <int N1, int N2>
+__global__ __launch_bounds__(1024) void k_lig_plane_split(ScanGeom g, const float2* __restrict__ twN1,
+                                                          const float2* __restrict__ twM2,
+                                                          const float2* __restrict__ tw2,
+                                                          const int* __restrict__ cells, int n_atoms,
+                                                          const float* __restrict__ w, float2* __restrict__ L1) {
+  constexpr int NT = 1024, NW = NT / 64;
+  constexpr int A1 = 16, B1 = NW;
+  static_assert(A1 * B1 == N1, "y split");
+  constexpr int M2 = N2 / 2, H2 = M2 + 1, LS = H2 | 1, NS = M2 / 64;
+  static_assert(M2 % 64 == 0, "whole column sets");
+  using SZ = Split<M2>;
+  constexpr int A2 = SZ::A, B2 = SZ::B, J2 = SZ::J;
+  constexpr int LZ = NT / B2;
+  static_assert(LZ == kPlaneSplitRows && LZ % 64 == 0, "one z pass over the box rows");
+  extern __shared__ float2 buf[];
+  float* fb = reinterpret_cast<float*>(buf);
+  __shared__ float2 s_nyq[N1];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int p = blockIdx.x, b = blockIdx.y, t = blockIdx.z;
+  const int i0 = g.lo[0] + p, lo1 = g.lo[1], p1 = g.p[1];
+  const int plane = N1 * N2;
+  const int gz = tid / LZ, rz = tid % LZ;
+  const int gzu = __builtin_amdgcn_readfirstlane(gz);
+  for (int e = tid; e < p1 * LS; e += NT) buf[e] = make_float2(0.f, 0.f);
+  __syncthreads();
+  {
+    const int* cb = cells + (size_t)b * n_atoms;
+    const float* wt = w + (size_t)t * n_atoms;
+    for (int j = tid; j < n_atoms; j += NT) {
+      const int off = cb[j];
+      if (off >= 0 && off / plane == i0) {
+        const int r = off - i0 * plane;
+        const int i1 = r / N2, i2 = r - i1 * N2;
+        atomicAdd(&fb[(i1 - lo1) * 2 * LS + i2], wt[j]);
+      }
+    }
+  }
+  __syncthreads();
+  // z pass: row rz of the box, 8 lanes
+  {
+    const bool rok = rz < p1;
+    float2 z[A2];
+    if (rok) {
+#pragma unroll
+      for (int m = 0; m < A2; ++m) z[m] = buf[rz * LS + gz + B2 * m];
+      four_step_phase1<A2, B2, -1>(z, gzu, twM2);
+    }
+    __syncthreads();
+    if (rok)
+#pragma unroll
+      for (int k1 = 0; k1 < A2; ++k1) buf[rz * LS + k1 * B2 + gz] = z[k1];
+    __syncthreads();
+    float2 u[J2][B2];
+#pragma unroll
+    for (int j = 0; j < J2; ++j) {
+      const int k1 = gz + B2 * j;
+      if (rok && k1 < A2)
+#pragma unroll
+        for (int q = 0; q < B2; ++q) u[j][q] = buf[rz * LS + k1 * B2 + q];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < J2; ++j) {
+      const int k1 = gz + B2 * j;
+      if (rok && k1 < A2) {
+        rdft<B2, -1>(u[j]);
+#pragma unroll
+        for (int k2 = 0; k2 < B2; ++k2) buf[rz * LS + k1 + A2 * k2] = u[j][k2];
+      }
+    }
+    __syncthreads();
+  }
+  // r2c post-step in place: X[0] = Re Z0 + Im Z0 and X[M2] = Re Z0 - Im Z0
+  // (both real), X[k] and X[M2 - k] from the pair Z[k], Z[M2 - k]
+  constexpr int NP = M2 / 2 + 1;
+  for (int e = tid; e < p1 * NP; e += NT) {
+    const int rr = e / NP, k = e - rr * NP;
+    const float2 zk = buf[rr * LS + k], zm = buf[rr * LS + (M2 - k) % M2];
+    if (k == 0) {
+      buf[rr * LS] = make_float2(zk.x + zk.y, 0.f);
+      buf[rr * LS + M2] = make_float2(zk.x - zk.y, 0.f);
+    } else {
+      buf[rr * LS + k] = r2c_point(zk, zm, tw2[k]);
+      if (k != M2 - k) buf[rr * LS + M2 - k] = r2c_point(zm, zk, tw2[M2 - k]);
+    }
+  }
+  __syncthreads();
+  // stores through a descriptor over this plane: 32-bit offsets
+  const __amdgpu_buffer_rsrc_t rd =
+      pd_rsrc(L1 + (((size_t)b * g.T + t) * g.pp + (g.lo[0] - g.pl) + p) * (size_t)(N1 * H2), N1 * H2 * 8);
+  if (wave == 0 && lane < B1) {
+    // column kz = M2: the same four-step on 16 lanes, the exchange inside the
+    // wave
+    float2 v[A1];
+#pragma unroll
+    for (int m = 0; m < A1; ++m) {
+      const int rr = lane + B1 * m - lo1;
+      v[m] = (rr >= 0 && rr < p1) ? buf[rr * LS + M2] : make_float2(0.f, 0.f);
+    }
+    four_step_phase1<A1, B1, -1>(v, lane, twN1);
+#pragma unroll
+    for (int k1 = 0; k1 < A1; ++k1) s_nyq[k1 * B1 + lane] = v[k1];
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int q = 0; q < B1; ++q) v[q] = s_nyq[lane * B1 + q];
+    rdft<B1, -1>(v);
+#pragma unroll
+    for (int k2 = 0; k2 < B1; ++k2) pd_st2(rd, (uint32_t)((lane * H2 + M2) * 8), (uint32_t)(A1 * k2 * H2 * 8), v[k2]);
+  }
+  float2 x[NS][A1];
+#pragma unroll
+  for (int s = 0; s < NS; ++s)
+#pragma unroll
+    for (int m = 0; m < A1; ++m) {
+      const int rr = wave + B1 * m - lo1;  // wave-uniform
+      x[s][m] = (rr >= 0 && rr < p1) ? buf[rr * LS + lane + 64 * s] : make_float2(0.f, 0.f);
+    }
+  __syncthreads();  // the rows are read: the buffer takes the y exchange
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    rdft<A1, -1>(x[s]);
+#pragma unroll
+    for (int k1 = 1; k1 < A1; ++k1) x[s][k1] = cmul(x[s][k1], twN1[wave * k1]);
+    if (s > 0) __syncthreads();
+#pragma unroll
+    for (int k1 = 0; k1 < A1; ++k1) buf[(k1 * B1 + wave) * 64 + lane] = x[s][k1];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < B1; ++q) x[s][q] = buf[(wave * B1 + q) * 64 + lane];
+    rdft<B1, -1>(x[s]);
+#pragma unroll
+    for (int k2 = 0; k2 < B1; ++k2)
+      pd_st2(rd, (uint32_t)((lane + 64 * s) * 8), (uint32_t)((wave + A1 * k2) * H2 * 8), x[s][k2]);
+  }
+}

@@ -66,6 +66,7 @@ class ScanDesc(C.Structure):
 SCAN_SPLIT_PLANES = 1
 SCAN_ONE_LANE = 2
 SCAN_SPLIT_INVERSE = 4
+SCAN_SPLIT_FORWARD = 8
 
 
 class Term(C.Structure):

@@ -93,10 +93,11 @@ CASES_BOTH = [c + (0,) for c in CASES] + [
     ((96, 96, 96), 1.2, 1500, 300, 4, SPLIT),
     # z half-length 26 has no register-FFT split: the runtime-plan row kernel
     ((56, 44, 52), 1.5, 300, 80, 2, SPLIT),
-    # N1 = N2 = 256 (a 256 x 129 plane exceeds LDS): the inverse y pass, rows and
-    # selection in one kernel, and the same case through the two inverse kernels
+    # N1 = N2 = 256 (a 256 x 129 plane exceeds LDS): the ligand plane pass and the
+    # inverse (y pass, rows, selection) each in one kernel, and the same case
+    # through the separate row and column kernels
     ((32, 256, 256), 1.5, 300, 80, 2, 0),
-    ((32, 256, 256), 1.5, 300, 80, 2, pdfft.SCAN_SPLIT_INVERSE),
+    ((32, 256, 256), 1.5, 300, 80, 2, pdfft.SCAN_SPLIT_INVERSE | pdfft.SCAN_SPLIT_FORWARD),
 ]
 
 
