@@ -582,12 +582,9 @@ __global__ __launch_bounds__(1024) void k_inv_plane_topk_split(ScanGeom g, const
       }
     __syncthreads();
     const int row = LZ * hh + rz;
-    // c2r pre-step in two groups, so that only half of its loads are in
-    // flight next to the rows held for the second half
     float2 z[A2];
 #pragma unroll
     for (int m = 0; m < A2; ++m) {
-      if (m == A2 / 2) __builtin_amdgcn_sched_barrier(0);
       const int k = gz + B2 * m;
       float2 xk = buf[rz * LS + k], xm = buf[rz * LS + M2 - k];
       if (k == 0) { xk.y = 0.f; xm.y = 0.f; }
@@ -605,7 +602,6 @@ __global__ __launch_bounds__(1024) void k_inv_plane_topk_split(ScanGeom g, const
 #pragma unroll
     for (int j = 0; j < J2; ++j) {
       const int k1 = gz + B2 * j;
-      if (j > 0) __builtin_amdgcn_sched_barrier(0);
       if (k1 < A2) {
         float2 u[B2];
 #pragma unroll
@@ -667,11 +663,13 @@ __global__ __launch_bounds__(1024) void k_inv_plane_topk_split(ScanGeom g, const
           ++popped;
         }
         if (h[0] == kNoKey && popped > 0 && popped % L == 0) {
+          // rescan (rare): a rolled loop, so that the values are not held in
+          // registers next to the rows of the next half
 #pragma unroll
           for (int j = 0; j < J2; ++j) {
             const int k1 = gz + B2 * j;
             if (k1 < A2) {
-#pragma unroll
+#pragma unroll 1
               for (int k2 = 0; k2 < B2; ++k2) {
                 const int e = base + row * N2 + 2 * (k1 + A2 * k2);
                 const float2 v = buf[rz * LS + k1 * B2 + k2];
