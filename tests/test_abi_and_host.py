@@ -42,6 +42,15 @@ def test_peak_struct_layout_matches_header():
     assert pdfft.PEAK_DTYPE.itemsize == 40
 
 
+def test_scan_flags_match_header():
+    text = open(os.path.join(ROOT, "include", "pdfft.h")).read()
+    flags = {k: int(v) for k, v in re.findall(r"#define PD_SCAN_([A-Z_]+) (\d+)", text)}
+    assert set(flags) == {"SPLIT_PLANES", "ONE_LANE", "SPLIT_INVERSE", "SPLIT_FORWARD"}
+    for name, value in flags.items():
+        assert getattr(pdfft, "SCAN_" + name) == value, name
+    assert len(set(flags.values())) == len(flags) and all(v & (v - 1) == 0 for v in flags.values())
+
+
 def test_synthetic_globule_constraints():
     cx = synth.make_complex(1500, 450, seed=3)
     for mol, n in ((cx.receptor, 1500), (cx.ligand, 450)):
