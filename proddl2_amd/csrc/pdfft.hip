@@ -450,7 +450,7 @@ pd_status launch_fft_path(pd_scan* h, const pd_scan::Lane& ln, int nb, const dou
                            dim3(kSplitThreads), split_col_lds(h), stream, g, h->py, ln.L1, g.p[0], g.pp,
                            g.lo[0] - g.pl, g.lo[1], g.lo[1] + g.p[1]);
     } else if (f)
-      hipLaunchKernelGGL(f->lig, dim3(g.p[0], nb), dim3(kPlaneThreads), plane_lds(h), stream, g,
+      hipLaunchKernelGGL(f->lig, dim3((g.p[0] + 15) / 16 * 16, nb), dim3(kPlaneThreads), plane_lds(h), stream, g,
                          h->tw_y, h->tw_z, h->tw2, ln.cells, h->n_lig, h->w32, ln.L1);
     else
       hipLaunchKernelGGL(k_lig_plane_fwd, dim3(g.p[0], nb), dim3(kPlaneThreads), plane_lds(h), stream,

@@ -504,7 +504,16 @@ __global__ __launch_bounds__(512, PD_LIG_WAVES) void k_lig_plane_fwd_fast(ScanGe
   extern __shared__ float2 buf[];
   float* fb = reinterpret_cast<float*>(buf);
   const int tid = threadIdx.x;
-  const int p = blockIdx.x, b = blockIdx.y;
+  // Blocks go to the 8 XCDs round-robin in launch order.  The 16 planes of one
+  // interleaved group share each 512-byte block of L1 in 32-byte pieces, so
+  // they run on one XCD, whose L2 then merges the pieces into whole lines:
+  // block L (on XCD L % 8) takes item ((L / 128) * 8 + L % 8) * 16 + (L / 8) % 16
+  // of the plane-fastest items over a plane count padded to 16 (gridDim.x);
+  // a last partial group of 128 blocks keeps item L.
+  const int PX = gridDim.x, Lb = blockIdx.x + PX * blockIdx.y, nblk = PX * gridDim.y;
+  const int item = Lb < nblk / 128 * 128 ? ((Lb >> 7) * 8 + (Lb & 7)) * 16 + ((Lb >> 3) & 15) : Lb;
+  const int p = item % PX, b = item / PX;
+  if (p >= g.p[0]) return;  // padding plane: the whole block leaves before any barrier
   const int i0 = g.lo[0] + p;
   const int lo1 = g.lo[1], p1 = g.p[1];
   const int plane = N1 * N2;
