@@ -434,8 +434,8 @@ pd_status launch_fft_path(pd_scan* h, const pd_scan::Lane& ln, int nb, const dou
   const FastSet* f = h->fast;
   {
     Timer tm(h, 1, stream);
-    if (h->split && h->lig_fused && g.p[1] <= kPlaneSplitRows && g.il == 0) {
-      hipLaunchKernelGGL(kLigSplit, dim3(g.p[0], nb, g.T), dim3(1024), kInvSplitLds, stream, g, h->tw_y, h->tw_z, h->tw2,
+    if (h->split && h->lig_fused && g.p[1] <= kPlaneSplitRows) {
+      hipLaunchKernelGGL(kLigSplit, dim3((g.p[0] + 15) / 16 * 16, nb, g.T), dim3(1024), kInvSplitLds, stream, g, h->tw_y, h->tw_z, h->tw2,
                          ln.cells, h->n_lig, h->w32, ln.L1);
     } else if (h->split) {
       hipLaunchKernelGGL(k_lig_rows, dim3(g.p[0], nb, (g.p[1] + kSplitRows - 1) / kSplitRows),
@@ -896,12 +896,15 @@ pd_status pd_scan_set_ligand(pd_scan* h, int n_atoms, const double* pos, const d
   const int al = h->fcol ? h->fcol->lanes : 1;
   g.pl = g.lo[0];
   g.pp = std::min(g.n0, (g.p[0] + al - 1) / al * al);
-  // Interleaved L1 order for the fast column pass (whole-plane kernels only;
-  // the split path transforms L1 planes in place): needs whole 64/B column
-  // blocks and a group that its 32-bit offsets cover.
+  // Interleaved L1 order for the fast column pass, where one kernel writes
+  // each ligand plane (the whole-plane kernels, or the one-kernel ligand pass
+  // of the split path; the split row and column kernels transform L1 planes
+  // in place): needs whole 64/B column blocks and a group that its 32-bit
+  // offsets cover.
   g.il = 0;
   g.ilsh = 0;
-  if (h->fcol && !h->split && g.M % (64 / al) == 0 && (long long)g.pp * g.M * 8 <= 0xFFFFFFFFLL) {
+  const bool one_writer = !h->split || (h->lig_fused && g.p[1] <= kPlaneSplitRows);
+  if (h->fcol && one_writer && g.M % (64 / al) == 0 && (long long)g.pp * g.M * 8 <= 0xFFFFFFFFLL) {
     g.il = al;
     while ((1 << g.ilsh) < 64 / al) ++g.ilsh;
   }

@@ -719,7 +719,7 @@ __global__ __launch_bounds__(1024) void k_inv_plane_topk_split(ScanGeom g, const
 // ---------------------------------------------------------------------------
 // Ligand plane pass in one kernel for N1 = 256 and N2 = 256 with at most 128
 // box rows: the plane's spectrum is written once, as whole rows.
-// grid (p0, nrot, T), block 1024 = 16 waves; one term per block:
+// grid (p0 padded to 16, nrot, T), block 1024 = 16 waves; one term per block:
 //   1. deposit the plane's atoms into the box rows (real rows in LDS, row
 //      stride LS complex);
 //   2. packed half-length forward z FFT of each box row (register four-step,
@@ -727,8 +727,11 @@ __global__ __launch_bounds__(1024) void k_inv_plane_topk_split(ScanGeom g, const
 //   3. r2c post-step in place (X[k] and X[M2 - k] from Z[k] and Z[M2 - k]);
 //   4. the lanes load their column sets (lane c, wave r: kz = c + 64 s, rows
 //      r + 16 m, zero outside the box), forward y FFT four-step 16 x 16 with
-//      one LDS exchange per set, stores of 512 contiguous bytes per wave.
-//      Column kz = M2 is transformed by lanes 0..15 of wave 0 alone.
+//      one LDS exchange per set; a wave's stores cover 512 contiguous bytes
+//      of a row (plane-major) or 32-byte pieces of the column pass's
+//      interleaved order (g.il), whose 16-plane groups run on one XCD so that
+//      its L2 merges the pieces into whole lines.  Column kz = M2 is
+//      transformed by lanes 0..15 of wave 0 alone.
 // LDS: 128 * LS complex (the box rows, then the y exchange).
 // ---------------------------------------------------------------------------
 constexpr int kPlaneSplitRows = 128;  // box rows the one-kernel ligand pass holds
@@ -753,7 +756,15 @@ __global__ __launch_bounds__(1024) void k_lig_plane_split(ScanGeom g, const floa
   __shared__ float2 s_nyq[N1];
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int p = blockIdx.x, b = blockIdx.y, t = blockIdx.z;
+  // blocks go to the 8 XCDs round-robin in launch order: block L (on XCD
+  // L % 8) takes item ((L / 128) * 8 + L % 8) * 16 + (L / 8) % 16 of the
+  // plane-fastest items, so each group of 16 planes stays on one XCD (a last
+  // partial group of 128 blocks keeps item L)
+  const int PX = gridDim.x, Lb = blockIdx.x + PX * (blockIdx.y + gridDim.y * blockIdx.z);
+  const int nblk = PX * gridDim.y * gridDim.z;
+  const int item = Lb < nblk / 128 * 128 ? ((Lb >> 7) * 8 + (Lb & 7)) * 16 + ((Lb >> 3) & 15) : Lb;
+  const int p = item % PX, b = (item / PX) % gridDim.y, t = item / (PX * gridDim.y);
+  if (p >= g.p[0]) return;  // padding plane: the whole block leaves before any barrier
   const int i0 = g.lo[0] + p, lo1 = g.lo[1], p1 = g.p[1];
   const int plane = N1 * N2;
   const int gz = tid / LZ, rz = tid % LZ;
@@ -822,9 +833,16 @@ __global__ __launch_bounds__(1024) void k_lig_plane_split(ScanGeom g, const floa
     }
   }
   __syncthreads();
-  // stores through a descriptor over this plane: 32-bit offsets
-  const __amdgpu_buffer_rsrc_t rd =
-      pd_rsrc(L1 + (((size_t)b * g.T + t) * g.pp + (g.lo[0] - g.pl) + p) * (size_t)(N1 * H2), N1 * H2 * 8);
+  // stores through a descriptor over the (rotation, term)'s planes, 32-bit
+  // offsets; column m of plane q sits at l1_off(g, q, m)
+  constexpr int PM = N1 * H2;
+  const __amdgpu_buffer_rsrc_t rd = pd_rsrc(L1 + ((size_t)b * g.T + t) * g.pp * (size_t)PM, (uint32_t)g.pp * PM * 8u);
+  const int q = g.lo[0] - g.pl + p;
+  const uint32_t qb = g.il ? (uint32_t)((q / g.il) * (PM >> g.ilsh) * 64 + (q % g.il) * (64 / g.il)) : (uint32_t)q * PM;
+  const uint32_t cmask = g.il ? (64u / g.il) - 1u : 0u;
+  auto off8 = [&](int m) -> uint32_t {  // byte offset of column m of this plane
+    return g.il ? (qb + ((uint32_t)(m >> g.ilsh) << 6) + ((uint32_t)m & cmask)) * 8u : (qb + (uint32_t)m) * 8u;
+  };
   if (wave == 0 && lane < B1) {
     // column kz = M2: the same four-step on 16 lanes, the exchange inside the
     // wave
@@ -842,7 +860,7 @@ __global__ __launch_bounds__(1024) void k_lig_plane_split(ScanGeom g, const floa
     for (int q = 0; q < B1; ++q) v[q] = s_nyq[lane * B1 + q];
     rdft<B1, -1>(v);
 #pragma unroll
-    for (int k2 = 0; k2 < B1; ++k2) pd_st2(rd, (uint32_t)((lane * H2 + M2) * 8), (uint32_t)(A1 * k2 * H2 * 8), v[k2]);
+    for (int k2 = 0; k2 < B1; ++k2) pd_st2(rd, off8((lane + A1 * k2) * H2 + M2), 0u, v[k2]);
   }
   float2 x[NS][A1];
 #pragma unroll
@@ -866,7 +884,6 @@ __global__ __launch_bounds__(1024) void k_lig_plane_split(ScanGeom g, const floa
     for (int q = 0; q < B1; ++q) x[s][q] = buf[(wave * B1 + q) * 64 + lane];
     rdft<B1, -1>(x[s]);
 #pragma unroll
-    for (int k2 = 0; k2 < B1; ++k2)
-      pd_st2(rd, (uint32_t)((lane + 64 * s) * 8), (uint32_t)((wave + A1 * k2) * H2 * 8), x[s][k2]);
+    for (int k2 = 0; k2 < B1; ++k2) pd_st2(rd, off8((wave + A1 * k2) * H2 + lane + 64 * s), 0u, x[s][k2]);
   }
 }
