@@ -15,6 +15,7 @@ per launch, and then the number of concurrent lanes of the graph, by timing
 them on the device.
 
     python -m proddl2_amd.batch            # prints one JSON line
+    python -m proddl2_amd.batch --no-graph # eager only: tuned rotations per launch, no capture
 """
 from __future__ import annotations
 
@@ -109,7 +110,8 @@ def make_graph(pairs, n_lanes):
 
 def autotune(pairs, torch, batches=(32, 64, 128), lanes=(1, 2, 4), sample=640, log=print):
     """Per edge: rotations per launch with the lowest time per rotation on a
-    sample of the first pair of that edge; then the lane count of the graph."""
+    sample of the first pair of that edge; then the lane count of the graph
+    (skipped when ``lanes`` is empty; the lane count returned is then 0)."""
     tuned = {}
     for e in sorted({p.spec["edge"] for p in pairs}):
         p = next(q for q in pairs if q.spec["edge"] == e)
@@ -128,7 +130,7 @@ def autotune(pairs, torch, batches=(32, 64, 128), lanes=(1, 2, 4), sample=640, l
         for q in pairs:
             if q.spec["edge"] == e:
                 q.scan.set_batch(best[0])
-    best_l = None
+    best_l = (0, None)
     for nl in lanes:
         g = make_graph(pairs, nl)
         g.launch()
@@ -139,7 +141,7 @@ def autotune(pairs, torch, batches=(32, 64, 128), lanes=(1, 2, 4), sample=640, l
         dt = time.perf_counter() - t0
         g.close()
         log(f"autotune lanes {nl}: {dt * 1e3:.1f} ms per replay")
-        if best_l is None or dt < best_l[1]:
+        if best_l[1] is None or dt < best_l[1]:
             best_l = (nl, dt)
     return tuned, best_l[0]
 
@@ -150,6 +152,8 @@ def main(argv=None) -> int:
     ap.add_argument("--rotations", type=int, default=5000)
     ap.add_argument("--seed", type=int, default=3000)
     ap.add_argument("--replays", type=int, default=3)
+    ap.add_argument("--no-graph", action="store_true",
+                    help="eager only: tune rotations per launch, then time the pairs one after another")
     a = ap.parse_args(argv)
     log = lambda m: print(f"[batch] {m}", file=sys.stderr, flush=True)  # noqa: E731
     import torch
@@ -171,7 +175,30 @@ def main(argv=None) -> int:
     t_eager = time.perf_counter() - t0
     eager = [p.peaks() for p in pairs]
     log(f"eager, pair by pair (batch 64): {t_eager * 1e3:.1f} ms, {total / t_eager:.0f} rotations/s")
-    tuned, n_lanes = autotune(pairs, torch, log=log)
+    tuned, n_lanes = autotune(pairs, torch, lanes=() if a.no_graph else (1, 2, 4), log=log)
+    if a.no_graph:
+        for p in pairs:
+            p.run()
+        t0 = time.perf_counter()
+        for p in pairs:
+            p.run()
+        t_tuned = time.perf_counter() - t0
+        same = all(np.array_equal(p.peaks()[1], no) and np.array_equal(p.peaks()[0].view(np.uint8), pk.view(np.uint8))
+                   for p, (pk, no) in zip(pairs, eager))
+        log(f"eager, tuned rotations per launch: {t_tuned * 1e3:.1f} ms, {total / t_tuned:.0f} rotations/s; "
+            f"identical: {same}")
+        print(json.dumps({
+            "workload": f"batched-uneven: {len(pairs)} pairs, receptor 800-6000 / ligand 200-2000 atoms "
+                        f"(log-uniform, seed {a.seed}), {a.rotations} rotations each, 4 terms, h from {STEPS} A",
+            "pairs": len(pairs), "rotations": total,
+            "edges": {int(k): int(v) for k, v in zip(*np.unique([s["edge"] for s in specs], return_counts=True))},
+            "eager_pair_by_pair_s": t_eager, "eager_tuned_s": t_tuned,
+            "rotations_per_s_eager": total / t_eager, "rotations_per_s_eager_tuned": total / t_tuned,
+            "tuned_batch_per_edge": {int(k): int(v) for k, v in tuned.items()}, "graph": None,
+            "tuned_identical_to_eager": same}), flush=True)
+        for p in pairs:
+            p.close()
+        return 0 if same else 1
     for p in pairs:
         p.d_peaks.zero_()
         p.d_nout.zero_()

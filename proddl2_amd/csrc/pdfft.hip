@@ -243,6 +243,7 @@ struct pd_scan {
   const FastYCols* fycol = nullptr;  // split y passes specialised on N1, if any
   bool inv_fused = false;            // split inverse as one kernel (N1 = N2 = 256)
   bool lig_fused = false;            // split ligand pass as one kernel (N1 = N2 = 256)
+  bool lig_rows = false;             // whole-plane grid whose ligand pass runs as row + column kernels
   bool split = false;  // plane passes as row + column kernels (scan_split.hpp)
   HostPlan hx, hy, hz;
   FftPlan px{}, py{}, pz{};
@@ -437,7 +438,7 @@ pd_status launch_fft_path(pd_scan* h, const pd_scan::Lane& ln, int nb, const dou
     if (h->split && h->lig_fused && g.p[1] <= kPlaneSplitRows) {
       hipLaunchKernelGGL(kLigSplit, dim3((g.p[0] + 15) / 16 * 16, nb, g.T), dim3(1024), kInvSplitLds, stream, g, h->tw_y, h->tw_z, h->tw2,
                          ln.cells, h->n_lig, h->w32, ln.L1);
-    } else if (h->split) {
+    } else if (h->split || h->lig_rows) {
       hipLaunchKernelGGL(k_lig_rows, dim3(g.p[0], nb, (g.p[1] + kSplitRows - 1) / kSplitRows),
                          dim3(kSplitThreads), split_row_lds(h), stream, g, h->pz, h->tw2, ln.cells, h->n_lig,
                          h->w32, ln.L1);
@@ -729,6 +730,21 @@ pd_status pd_scan_create(const pd_scan_desc* desc, pd_scan** out) {
     h->py = FftPlan{g.n1, ny, h->d_stages + nx, h->tw_y};
     h->pz = FftPlan{g.m2, nz, h->d_stages + nx + ny, h->tw_z};
   }
+  // A whole-plane grid whose plane takes more than half of LDS (one block per
+  // CU: 144^3, 160^3) runs the ligand pass as the split row and column
+  // kernels, which measured twice as fast there (160^3: 32.9 -> 15.4 us per
+  // rotation); its inverse keeps the whole-plane kernel.
+  h->lig_rows = !h->split && 2 * plane_lds(h) > kMaxLds && plan_fits(h->hz, kSplitRows, kSplitThreads) &&
+                plan_fits(h->hy, kSplitCols, kSplitThreads) && split_row_lds(h) <= kMaxLds &&
+                split_col_lds(h) <= kMaxLds;
+  if (h->lig_rows) {
+    if ((st = set_lds((const void*)k_lig_rows, split_row_lds(h))) != PD_OK ||
+        (st = set_lds((const void*)k_cols_fft<-1>, split_col_lds(h))) != PD_OK)
+      return bad(st, pd_detail::last_error());
+    h->fycol = find_fast_ycols(g.n1);
+    if (h->fycol && (st = set_lds((const void*)h->fycol->fwd, h->fycol->lds)) != PD_OK)
+      return bad(st, pd_detail::last_error());
+  }
   if (h->split) {
     const void* rows[] = {(const void*)k_lig_rows, (const void*)k_rec_rows, (const void*)k_inv_rows_topk};
     for (const void* fn : rows)
@@ -897,13 +913,14 @@ pd_status pd_scan_set_ligand(pd_scan* h, int n_atoms, const double* pos, const d
   g.pl = g.lo[0];
   g.pp = std::min(g.n0, (g.p[0] + al - 1) / al * al);
   // Interleaved L1 order for the fast column pass, where one kernel writes
-  // each ligand plane (the whole-plane kernels, or the one-kernel ligand pass
-  // of the split path; the split row and column kernels transform L1 planes
-  // in place): needs whole 64/B column blocks and a group that its 32-bit
-  // offsets cover.
+  // each ligand plane (the whole-plane ligand kernel, or the one-kernel
+  // ligand pass of the split path; the row and column kernels transform L1
+  // planes in place): needs whole 64/B column blocks and a group that its
+  // 32-bit offsets cover.
   g.il = 0;
   g.ilsh = 0;
-  const bool one_writer = !h->split || (h->lig_fused && g.p[1] <= kPlaneSplitRows);
+  const bool one_writer =
+      h->split ? h->lig_fused && g.p[1] <= kPlaneSplitRows : !h->lig_rows;
   if (h->fcol && one_writer && g.M % (64 / al) == 0 && (long long)g.pp * g.M * 8 <= 0xFFFFFFFFLL) {
     g.il = al;
     while ((1 << g.ilsh) < 64 / al) ++g.ilsh;

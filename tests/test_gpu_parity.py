@@ -70,6 +70,10 @@ CASES = [
     # edge-specialised column pass (N0 = 64) behind the generic plane kernels, with the
     # ligand spectra in the column pass's interleaved order
     ((64, 48, 40), 1.5, 300, 80, 4),
+    # planes over half of LDS: the ligand pass as row and column kernels, the
+    # inverse as the whole-plane kernel
+    ((144, 144, 144), 1.2, 2000, 500, 4),
+    ((160, 160, 160), 1.2, 2000, 500, 2),
 ]
 
 
