@@ -74,6 +74,8 @@ CASES = [
     # inverse as the whole-plane kernel
     ((144, 144, 144), 1.2, 2000, 500, 4),
     ((160, 160, 160), 1.2, 2000, 500, 2),
+    # the same choice behind the runtime-plan kernels (150 x 76 plane: 92 KB of LDS)
+    ((40, 150, 150), 1.5, 300, 80, 2),
 ]
 
 
