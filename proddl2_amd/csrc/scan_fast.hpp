@@ -29,13 +29,19 @@ struct Split {
   static constexpr int J = B ? (A + B - 1) / B : 0;
 };
 
-// Lanes per column of the column pass (16 lanes when PD_COL_B16 and N = 16 A, A >= 4).
+// Lanes per column of the column pass: 16 lanes (N = 16 A, 4 <= A <= 16) where
+// the split of the plane passes would give each lane three or more rounds of
+// B values (144, 160: 8 lanes x 18 or 20 values), or everywhere with
+// PD_COL_B16.  Measured: 16 lanes take the 160^3 column pass from 20.2 to
+// 14.2 us per rotation and 144^3 from 15.9 to 10.9, but slow 64^3-128^3
+// (128^3: 161 -> 242 ms per 30,000 rotations).
 #ifndef PD_COL_B16
 #define PD_COL_B16 0
 #endif
 template <int N>
 struct ColSplit {
-  static constexpr bool W = PD_COL_B16 && N % 16 == 0 && N / 16 >= 4 && N / 16 <= 16;
+  static constexpr bool W =
+      (PD_COL_B16 || Split<N>::J >= 3) && N % 16 == 0 && N / 16 >= 4 && N / 16 <= 16;
   static constexpr int B = W ? 16 : Split<N>::B;
   static constexpr int A = W ? N / 16 : Split<N>::A;
   static constexpr int J = B ? (A + B - 1) / B : 0;
