@@ -15,7 +15,8 @@ per launch, and then the number of concurrent lanes of the graph, by timing
 them on the device.
 
     python -m proddl2_amd.batch            # prints one JSON line
-    python -m proddl2_amd.batch --no-graph # eager only: tuned rotations per launch, no capture
+    python -m proddl2_amd.batch --no-graph # no capture: tuned rotations per launch, pairs one after
+                                           # another and all pairs enqueued at once
 """
 from __future__ import annotations
 
@@ -187,13 +188,34 @@ def main(argv=None) -> int:
                    for p, (pk, no) in zip(pairs, eager))
         log(f"eager, tuned rotations per launch: {t_tuned * 1e3:.1f} ms, {total / t_tuned:.0f} rotations/s; "
             f"identical: {same}")
+        # every pair enqueued on its own streams without waiting, one wait at the
+        # end: pairs overlap as the graph's lanes let them, without capture
+        for p in pairs:
+            p.d_peaks.zero_()
+            p.d_nout.zero_()
+        torch.cuda.synchronize()
+        for p in pairs:
+            p.run(sync=False)
+        for p in pairs:
+            p.scan.sync()
+        t0 = time.perf_counter()
+        for p in pairs:
+            p.run(sync=False)
+        for p in pairs:
+            p.scan.sync()
+        t_conc = time.perf_counter() - t0
+        same_c = all(np.array_equal(p.peaks()[1], no) and np.array_equal(p.peaks()[0].view(np.uint8), pk.view(np.uint8))
+                     for p, (pk, no) in zip(pairs, eager))
+        same = same and same_c
+        log(f"all pairs enqueued at once: {t_conc * 1e3:.1f} ms, {total / t_conc:.0f} rotations/s; identical: {same_c}")
         print(json.dumps({
             "workload": f"batched-uneven: {len(pairs)} pairs, receptor 800-6000 / ligand 200-2000 atoms "
                         f"(log-uniform, seed {a.seed}), {a.rotations} rotations each, 4 terms, h from {STEPS} A",
             "pairs": len(pairs), "rotations": total,
             "edges": {int(k): int(v) for k, v in zip(*np.unique([s["edge"] for s in specs], return_counts=True))},
-            "eager_pair_by_pair_s": t_eager, "eager_tuned_s": t_tuned,
+            "eager_pair_by_pair_s": t_eager, "eager_tuned_s": t_tuned, "all_enqueued_s": t_conc,
             "rotations_per_s_eager": total / t_eager, "rotations_per_s_eager_tuned": total / t_tuned,
+            "rotations_per_s_all_enqueued": total / t_conc,
             "tuned_batch_per_edge": {int(k): int(v) for k, v in tuned.items()}, "graph": None,
             "tuned_identical_to_eager": same}), flush=True)
         for p in pairs:
