@@ -51,8 +51,9 @@ const FastSet kFast[] = {PD_FAST(32),  PD_FAST(48),  PD_FAST(64),  PD_FAST(96),
 
 // Column kernels depend on N0 only: they serve every grid whose N0 is listed,
 // whatever its plane kernels (whole-plane, generic or split).  Each comes in
-// eight variants by the number MZ of x positions per lane that may be nonzero
-// (ceil(q A / 8), q = 1..8): the variant with the smallest MZ >= pp / B runs.
+// sixteen variants by the number MZ of x positions per lane that may be
+// nonzero (ceil(q A / 16), q = 1..16, so every MZ for A <= 16): the variant
+// with the smallest MZ >= pp / B runs.
 struct FastCol {
   int n;
   int col_lines;
@@ -60,23 +61,26 @@ struct FastCol {
   int a;      // A = n / B positions per lane
   int ni;     // receptor value pairs per lane and term (k_rhat_lanes)
   LanesFn lanes_fn;
-  int mz[8];
-  ColFn col[8];
+  int mz[16];
+  ColFn col[16];
 };
-#define PD_MZ(N, Q) ((ColSplit<N>::A * (Q) + 7) / 8)
+#define PD_MZ(N, Q) ((ColSplit<N>::A * (Q) + 15) / 16)
+#define PD_MZK(N, Q) k_col_fast<N, PD_MZ(N, Q)>
 #define PD_FCOL(N)                                                                                        \
   FastCol {                                                                                               \
     N, 256 / ColSplit<N>::B, ColSplit<N>::B, ColSplit<N>::A, ColLanes<N>::NI, k_rhat_lanes<N>,            \
-        {PD_MZ(N, 1), PD_MZ(N, 2), PD_MZ(N, 3), PD_MZ(N, 4), PD_MZ(N, 5), PD_MZ(N, 6), PD_MZ(N, 7),       \
-         PD_MZ(N, 8)},                                                                                    \
-        {k_col_fast<N, PD_MZ(N, 1)>, k_col_fast<N, PD_MZ(N, 2)>, k_col_fast<N, PD_MZ(N, 3)>,              \
-         k_col_fast<N, PD_MZ(N, 4)>, k_col_fast<N, PD_MZ(N, 5)>, k_col_fast<N, PD_MZ(N, 6)>,              \
-         k_col_fast<N, PD_MZ(N, 7)>, k_col_fast<N, PD_MZ(N, 8)>}                                          \
+        {PD_MZ(N, 1),  PD_MZ(N, 2),  PD_MZ(N, 3),  PD_MZ(N, 4),  PD_MZ(N, 5),  PD_MZ(N, 6),               \
+         PD_MZ(N, 7),  PD_MZ(N, 8),  PD_MZ(N, 9),  PD_MZ(N, 10), PD_MZ(N, 11), PD_MZ(N, 12),              \
+         PD_MZ(N, 13), PD_MZ(N, 14), PD_MZ(N, 15), PD_MZ(N, 16)},                                         \
+        {PD_MZK(N, 1),  PD_MZK(N, 2),  PD_MZK(N, 3),  PD_MZK(N, 4),  PD_MZK(N, 5),  PD_MZK(N, 6),         \
+         PD_MZK(N, 7),  PD_MZK(N, 8),  PD_MZK(N, 9),  PD_MZK(N, 10), PD_MZK(N, 11), PD_MZK(N, 12),        \
+         PD_MZK(N, 13), PD_MZK(N, 14), PD_MZK(N, 15), PD_MZK(N, 16)}                                      \
   }
 const FastCol kFastCol[] = {PD_FCOL(32),  PD_FCOL(48),  PD_FCOL(64),  PD_FCOL(96), PD_FCOL(120),
                             PD_FCOL(128), PD_FCOL(144), PD_FCOL(160), PD_FCOL(256)};
 #undef PD_FCOL
 #undef PD_MZ
+#undef PD_MZK
 
 // Row kernels of the split inverse specialised on N2.
 using RowsFn = void (*)(ScanGeom, const float2*, const float2*, const float2*, float, int, float*, int*, float*,
@@ -486,7 +490,7 @@ pd_status launch_fft_path(pd_scan* h, const pd_scan::Lane& ln, int nb, const dou
       const int nblk = 8 * ((ntile + 7) / 8) * nyb;
       const int mz = (g.pp + fc->lanes - 1) / fc->lanes;  // positions per lane that hold ligand planes
       int v = 0;
-      while (v < 7 && fc->mz[v] < mz) ++v;
+      while (v < 15 && fc->mz[v] < mz) ++v;
       hipLaunchKernelGGL(fc->col[v], dim3(nblk), dim3(kColThreads), 0, stream, g, h->tw_x, ln.L1, h->Rlanes,
                          ln.S1, bpb, nb, nyb);
     } else

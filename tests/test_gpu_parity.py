@@ -129,13 +129,13 @@ def test_correlation_grid(oracle, n, h, n_rec, n_lig, T, flags):
 
 # Ligand boxes of different widths select different column kernel variants:
 # MZ, the number of x positions per lane that may hold ligand planes (the
-# smallest of ceil(q A / 8), q = 1..8, covering the box), and with it the
-# zero-tail first DFT stage (MZ dividing A) or the full DFT (otherwise).
+# smallest of ceil(q A / 16), q = 1..16, covering the box: exact for A <= 16),
+# and with it the zero-tail first DFT stage (MZ dividing A) or the full DFT.
 COLUMN_VARIANTS = [
     ((32, 32, 32), 1.0, 200, 300, 2),       # A 4, MZ 4: full DFT
     ((64, 64, 64), 1.0, 400, 1500, 2),      # A 8, MZ 6: full DFT
-    ((96, 96, 96), 1.0, 400, 3000, 2),      # A 12, 7 of MZ 8 positions stored: one zero load
-    ((128, 128, 128), 1.2, 1500, 1500, 4),  # A 16, MZ 6: full DFT
+    ((96, 96, 96), 1.0, 400, 3000, 2),      # A 12, MZ 7: full DFT
+    ((128, 128, 128), 1.2, 1500, 1500, 4),  # A 16, MZ 5: full DFT
     ((128, 128, 128), 1.2, 1500, 800, 4),   # A 16, MZ 4: zero-tail DFT (R 4)
 ]
 
