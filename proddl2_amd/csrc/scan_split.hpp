@@ -533,7 +533,7 @@ __global__ __launch_bounds__(1024) void k_inv_plane_topk_split(ScanGeom g, const
     }
 #pragma unroll
     for (int k1 = 0; k1 < A1; ++k1) s_nyq[k1 * B1 + lane] = v[k1];
-    __builtin_amdgcn_wave_barrier();
+    wave_lds_sync();
 #pragma unroll
     for (int q = 0; q < B1; ++q) v[q] = s_nyq[lane * B1 + q];
     rdft<B1, +1>(v);
@@ -855,7 +855,7 @@ __global__ __launch_bounds__(1024) void k_lig_plane_split(ScanGeom g, const floa
     four_step_phase1<A1, B1, -1>(v, lane, twN1);
 #pragma unroll
     for (int k1 = 0; k1 < A1; ++k1) s_nyq[k1 * B1 + lane] = v[k1];
-    __builtin_amdgcn_wave_barrier();
+    wave_lds_sync();
 #pragma unroll
     for (int q = 0; q < B1; ++q) v[q] = s_nyq[lane * B1 + q];
     rdft<B1, -1>(v);
