@@ -344,6 +344,7 @@ __global__ __launch_bounds__(256, PD_COL_WAVES) void k_col_fast(ScanGeom g, cons
   const int m0 = tile * C;
   if (m0 >= M) return;
   const bool colok = m0 + col < M;
+  const bool sync_st = g.col_sync && m0 + C <= M;  // whole tile: no wave leaves early
   const int bbeg = rg * bpb, bend = min(nrot, bbeg + bpb);
   // L1 holds pp planes (a multiple of B) per (rotation, term): x position
   // lg + B m is a stored plane for every lane (m < pp / B) or for none, and
@@ -465,6 +466,7 @@ __global__ __launch_bounds__(256, PD_COL_WAVES) void k_col_fast(ScanGeom g, cons
     wave_lds_sync();
     rdft<A, +1>(x);
     const bool ok = rel < nrel;
+    if (sync_st) __syncthreads();
     const uint32_t sb = ok ? ((uint32_t)rel * g.n0 * M + m0) * 8u : 0u;
 #pragma unroll
     for (int m = 0; m < A; ++m) pd_st2(rs, ok ? (uint32_t)(lg + B * m) * M * 8u + vcol : sout, sb, x[m]);

@@ -46,6 +46,11 @@ struct ScanGeom {
   // B planes x 64/B columns, the order in which one wave of that pass reads a
   // position of its transform.  ilsh = log2(64 / B).
   int il, ilsh;
+  // Fast column pass: the block's four waves meet at a barrier before a
+  // rotation's stores, so the 32-byte pieces of each 128-byte line of S1
+  // reach L2 together (whole tiles only; set for N0 >= 256, where it was
+  // measured to pay).
+  int col_sync;
   double h;
   double zero_h[3];  // centred-geometry zero_ - h per axis
   double zsc[3];     // centred-geometry cell-0 centre per axis
