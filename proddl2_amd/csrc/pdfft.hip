@@ -687,8 +687,7 @@ pd_status pd_scan_create(const pd_scan_desc* desc, pd_scan** out) {
   g.M = g.n1 * g.h2;
   g.T = d.n_terms;
   g.h = d.grid_step;
-  // measured: 256^3 column pass 702 -> 606 ms per 10,000 rotations; at 128^3
-  // the whole pipeline ran 0.7% slower with it
+  // measured: 256^3 column pass 702 -> 606 ms per 10,000 rotations
   g.col_sync = g.n0 >= 256;
   for (int a = 0; a < 3; ++a) {
     centred_axis(d.grid_step, d.n[a], &g.zero_h[a], &g.zsc[a]);

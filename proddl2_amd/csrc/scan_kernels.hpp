@@ -47,9 +47,10 @@ struct ScanGeom {
   // position of its transform.  ilsh = log2(64 / B).
   int il, ilsh;
   // Fast column pass: the block's four waves meet at a barrier before a
-  // rotation's stores, so the 32-byte pieces of each 128-byte line of S1
-  // reach L2 together (whole tiles only; set for N0 >= 256, where it was
-  // measured to pay).
+  // rotation's stores, so that their pieces of each 128-byte line of S1 (and
+  // their next loads) go out together (whole tiles only).  Set for N0 >= 256:
+  // there the pass moves the same bytes 16% faster; at 144-160 it changed
+  // nothing and at 128 the pipeline ran 0.7% slower.
   int col_sync;
   double h;
   double zero_h[3];  // centred-geometry zero_ - h per axis
