@@ -11,10 +11,15 @@
 //                  1/N, and the plane's K best cells kept in a block-wide list
 //                  across the chunks).
 // Receptor:        k_rec_rows + k_cols_fft<-1>, then the usual k_col_fwd.
+// N1 = N2 = 256:   each plane pass in one 1,024-thread kernel that holds a
+//                  plane on chip in registers and LDS: k_lig_plane_split
+//                  (ligand boxes of up to 128 rows) and k_inv_plane_topk_split.
 //
-// Same arithmetic and the same selection rule ((value, offset) lexicographic,
-// NaN as +inf) as the whole-plane kernels of scan_kernels.hpp; one extra HBM
-// round trip of the plane per pass.
+// The same transforms (r2c along z, FFT along y, and their inverses; the FFT
+// factorisations differ, so results agree to rounding) and the same selection
+// rule ((value, offset) lexicographic, NaN as +inf) as the whole-plane kernels
+// of scan_kernels.hpp; the row and column kernels take one extra HBM round
+// trip of the plane per pass.
 #pragma once
 
 #include "scan_fast.hpp"
