@@ -476,13 +476,14 @@ pd_status launch_fft_path(pd_scan* h, const pd_scan::Lane& ln, int nb, const dou
     const int ntile = (g.M + C - 1) / C;
     int nyb = std::max(1, std::min(nb, 4096 / std::max(1, ntile)));
     if (fc) {
-      // One rotation pair per block, or two where that grid would pass ~32k
-      // blocks: the rotation groups of a tile run together on one XCD, so
-      // more groups share each read of the tile's receptor spectra in L2
-      // (measured: 128^3 column pass 188 -> 172 ms per 30,000 rotations,
-      // 256^3 843 -> 727 ms per 10,000).
-      const int per = (long long)ntile * ((nb + 1) / 2) <= 32768 ? 2 : 4;
-      nyb = std::max((nb + per - 1) / per, (int)((nb + max_bpb - 1) / max_bpb));
+      // About 4,096 blocks: the rotation groups of a tile run together on one
+      // XCD and share each read of the tile's receptor spectra through L2.
+      // Measured with three lanes: 128^3 (260 tiles) best at 16 groups of 4
+      // rotations (89.9k against 88.9k rotations/s with 32 pairs); 256^3
+      // (2,064 tiles) at 2 groups of 32 or 4 of 16 (column pass 525 / 523 ms
+      // per 10,000 rotations against 607 with 16 groups of 4).
+      nyb = std::max(1, std::min((nb + 1) / 2, (4096 + ntile / 2) / ntile));
+      nyb = std::max(nyb, (int)((nb + max_bpb - 1) / max_bpb));
     }
     int bpb = (nb + nyb - 1) / nyb;
     // the fast column pass takes rotations in pairs: an even group size
