@@ -476,13 +476,17 @@ pd_status launch_fft_path(pd_scan* h, const pd_scan::Lane& ln, int nb, const dou
     const int ntile = (g.M + C - 1) / C;
     int nyb = std::max(1, std::min(nb, 4096 / std::max(1, ntile)));
     if (fc) {
-      // About 4,096 blocks: the rotation groups of a tile run together on one
-      // XCD and share each read of the tile's receptor spectra through L2.
-      // Measured with three lanes: 128^3 (260 tiles) best at 16 groups of 4
-      // rotations (89.9k against 88.9k rotations/s with 32 pairs); 256^3
-      // (2,064 tiles) at 2 groups of 32 or 4 of 16 (column pass 525 / 523 ms
-      // per 10,000 rotations against 607 with 16 groups of 4).
-      nyb = std::max(1, std::min((nb + 1) / 2, (4096 + ntile / 2) / ntile));
+      // About 1,024 blocks: the rotation groups of a tile run together on one
+      // XCD and share each read of the tile's receptor spectra through L2,
+      // and fewer, longer blocks leave CUs to the other lanes' plane passes.
+      // Measured with three lanes at 128^3 (260 tiles): 4 groups of 16
+      // rotations 91.4-91.7k rotations/s, 8 groups 90.2-91.5k, 16 groups
+      // 89.7-90.4k, 2 groups 89.6-90.2k; the column pass timed alone is
+      // fastest with 16 groups (158 against 190 ms per 30,000 rotations).
+      // 256^3 (2,064 tiles) gets 2 groups of 32, the fewest its 32-bit
+      // offsets allow (column pass 525 ms per 10,000 rotations against 607
+      // with 16 groups of 4).
+      nyb = std::max(1, std::min((nb + 1) / 2, (1024 + ntile / 2) / ntile));
       nyb = std::max(nyb, (int)((nb + max_bpb - 1) / max_bpb));
     }
     int bpb = (nb + nyb - 1) / nyb;
