@@ -147,21 +147,29 @@ pd_status pd_scan_set_timing(struct pd_scan* h, int enable);
 pd_status pd_scan_kernel_stats(struct pd_scan* h, int k, double* total_ms, long long* launches);
 void* pd_scan_stream(struct pd_scan* h);
 
-void pd_scan_destroy(struct pd_scan* h);
+/* Frees the handle.  PD_ERR_STATE (and the handle stays valid) while a graph
+ * built from it exists. */
+pd_status pd_scan_destroy(struct pd_scan* h);
 const char* pd_last_error(void);
 
 /* Rotations per kernel launch (pd_scan_desc.batch), changed after creation;
- * reallocates the per-batch buffers.  Results do not depend on it. */
+ * reallocates the per-batch buffers.  Results do not depend on it.
+ * PD_ERR_STATE while a graph built from the handle exists (as for
+ * pd_scan_set_ligand, which also reallocates them). */
 pd_status pd_scan_set_batch(struct pd_scan* h, int batch);
 
-/* Several scans in one captured HIP graph.  No reference counterpart: the
- * reference runs one rotation range per process (python/lib/PRODDL/dock.py:86-113).
+/* Several scans in one HIP graph.  No reference counterpart: the reference
+ * runs one rotation range per process (python/lib/PRODDL/dock.py:86-113).
  * Pair i scans n_rot[i] device rotations d_rots[i] into d_peaks[i] / d_n_out[i]
  * (rot_index0 = 0) on lane lane[i] < n_lanes; the pairs of a lane run in the given
- * order, lanes run concurrently.  pd_graph_launch replays everything with one
- * launch (the buffers named at creation are reused); pd_graph_sync waits and
- * reports domain errors of any pair.  A scan may appear in one graph at a time
- * and must not run other work while its graph does. */
+ * order, lanes are independent branches.  The graph is built from kernel nodes
+ * with explicit dependencies (no stream capture); within a pair, batches go round
+ * the handle's batch buffers as in pd_scan_run_device.  pd_graph_launch replays
+ * everything with one launch (the buffers named at creation are reused);
+ * pd_graph_sync waits and reports domain errors of any pair.  A scan belongs to
+ * at most one graph; while it does, pd_scan_set_batch, pd_scan_set_ligand and
+ * pd_scan_destroy return PD_ERR_STATE, and work of its own (pd_scan_run...) must
+ * not overlap a replay. */
 pd_status pd_graph_create(int n, struct pd_scan* const* scans, const int* n_rot,
                           const double* const* d_rots, pd_peak* const* d_peaks,
                           int* const* d_n_out, const int* lane, int n_lanes,
@@ -180,6 +188,15 @@ pd_status pd_ljcomp_params(int n_rec, const double* eps_r, const double* sigma_r
                            const double* alpha_r, int n_lig, const double* eps_l,
                            const double* sigma_l, double* rep, double* att, double* w_rep,
                            double* w_att);
+
+/* Per-atom ljAvg parameters: MolForceLJ ctor (docking_molforce.hpp:377-450) with
+ * SoftCoreLJ and its shifted-force cutoff (potentials_inc.hpp:185-219): every
+ * receptor atom's LJ is mixed with the ligand's average eps and sigma (mix 0:
+ * arithmetic sigma, 1: geometric), params n_rec x 5 {e4s12, e4s6, as6, fshift,
+ * gshift} for PD_FIELD_LJ_FULL; ligand weights w (n_lig) are all 1. */
+pd_status pd_ljavg_params(int n_rec, const double* eps_r, const double* sigma_r, const double* alpha_r,
+                          int n_lig, const double* eps_l, const double* sigma_l, int mix, double cutoff,
+                          double* params, double* w);
 
 #ifdef __cplusplus
 }

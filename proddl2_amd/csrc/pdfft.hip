@@ -11,7 +11,10 @@
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <mutex>
 #include <string>
+#include <tuple>
+#include <utility>
 #include <vector>
 
 #include "../../include/pdfft.h"
@@ -235,6 +238,8 @@ bool plan_fits(const HostPlan& P, int lines, int threads) {
 
 }  // namespace
 
+struct pd_graph;
+
 struct pd_scan {
   pd_scan_desc d{};
   hipStream_t stream = nullptr;
@@ -293,6 +298,9 @@ struct pd_scan {
   std::vector<Pending> pending;
   double tot_ms[5] = {0, 0, 0, 0, 0};
   long long launches[5] = {0, 0, 0, 0, 0};
+  // The graph whose nodes name this scan's buffers, if any: while it exists the
+  // buffers must not move (set_batch, set_ligand and destroy are refused).
+  pd_graph* graph = nullptr;
 };
 
 namespace {
@@ -356,6 +364,45 @@ struct Timer {
       hipEventRecord(b, s);
       h->pending.push_back({k, a, b});
     }
+  }
+};
+
+// Where the kernels of a batch go: launched on a stream, or added as kernel
+// nodes to a graph under construction, each node after the chain's previous
+// node (the first after `deps`).  Arguments are converted to the kernel's
+// parameter types and copied into the node (hipGraphAddKernelNode copies them).
+struct Launcher {
+  hipStream_t s = nullptr;
+  hipGraph_t graph = nullptr;
+  std::vector<hipGraphNode_t> deps;
+  hipError_t err = hipSuccess;
+
+  template <typename... P, typename... A>
+  void operator()(void (*fn)(P...), dim3 grid, dim3 block, size_t shmem, A&&... args) {
+    static_assert(sizeof...(P) == sizeof...(A), "kernel argument count");
+    if (!graph) {
+      hipLaunchKernelGGL(fn, grid, block, shmem, s, std::forward<A>(args)...);
+      return;
+    }
+    std::tuple<std::decay_t<P>...> vals(std::forward<A>(args)...);
+    add_node(reinterpret_cast<void*>(fn), grid, block, shmem, vals, std::index_sequence_for<P...>{});
+  }
+
+ private:
+  template <typename Tup, size_t... I>
+  void add_node(void* fn, dim3 grid, dim3 block, size_t shmem, Tup& vals, std::index_sequence<I...>) {
+    void* ptrs[] = {static_cast<void*>(&std::get<I>(vals))...};
+    hipKernelNodeParams p{};
+    p.func = fn;
+    p.gridDim = grid;
+    p.blockDim = block;
+    p.sharedMemBytes = (unsigned)shmem;
+    p.kernelParams = ptrs;
+    p.extra = nullptr;
+    hipGraphNode_t node = nullptr;
+    if (err != hipSuccess) return;
+    err = hipGraphAddKernelNode(&node, graph, deps.empty() ? nullptr : deps.data(), deps.size(), &p);
+    if (err == hipSuccess) deps.assign(1, node);
   }
 };
 
@@ -429,39 +476,39 @@ pd_status receptor_spectra(pd_scan* h) {
 // pass, column pass, inverse plane pass + top-K).  grid_out (optional)
 // receives the scaled correlation grids.
 pd_status launch_fft_path(pd_scan* h, const pd_scan::Lane& ln, int nb, const double* d_rots,
-                          float* grid_out) {
-  hipStream_t stream = ln.s;
+                          float* grid_out, Launcher& L) {
+  hipStream_t stream = L.s;
   const ScanGeom& g = h->g;
   const float scale = (float)(1.0 / (double)h->N);
   {
     Timer tm(h, 0, stream);
     const long long tot = (long long)nb * h->n_lig;
-    hipLaunchKernelGGL(k_bin, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, stream, g, d_rots, nb,
+    L(k_bin, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, g, d_rots, nb,
                        h->q, h->n_lig, ln.cells, h->err, ln.thr);
   }
   const FastSet* f = h->fast;
   {
     Timer tm(h, 1, stream);
     if (h->split && h->lig_fused && g.p[1] <= kPlaneSplitRows) {
-      hipLaunchKernelGGL(kLigSplit, dim3((g.p[0] + 15) / 16 * 16, nb, g.T), dim3(1024), kInvSplitLds, stream, g, h->tw_y, h->tw_z, h->tw2,
+      L(kLigSplit, dim3((g.p[0] + 15) / 16 * 16, nb, g.T), dim3(1024), kInvSplitLds, g, h->tw_y, h->tw_z, h->tw2,
                          ln.cells, h->n_lig, h->w32, ln.L1);
     } else if (h->split || h->lig_rows) {
-      hipLaunchKernelGGL(k_lig_rows, dim3(g.p[0], nb, (g.p[1] + kSplitRows - 1) / kSplitRows),
-                         dim3(kSplitThreads), split_row_lds(h), stream, g, h->pz, h->tw2, ln.cells, h->n_lig,
+      L(k_lig_rows, dim3(g.p[0], nb, (g.p[1] + kSplitRows - 1) / kSplitRows),
+                         dim3(kSplitThreads), split_row_lds(h), g, h->pz, h->tw2, ln.cells, h->n_lig,
                          h->w32, ln.L1);
       if (h->fycol)
-        hipLaunchKernelGGL(h->fycol->fwd, dim3(nb * g.T * g.p[0], (g.h2 + h->fycol->cols - 1) / h->fycol->cols),
-                           dim3(256), h->fycol->lds, stream, g, h->tw_y, ln.L1, g.p[0], g.pp, g.lo[0] - g.pl,
+        L(h->fycol->fwd, dim3(nb * g.T * g.p[0], (g.h2 + h->fycol->cols - 1) / h->fycol->cols),
+                           dim3(256), h->fycol->lds, g, h->tw_y, ln.L1, g.p[0], g.pp, g.lo[0] - g.pl,
                            g.lo[1], g.lo[1] + g.p[1]);
       else
-        hipLaunchKernelGGL(k_cols_fft<-1>, dim3(nb * g.T * g.p[0], (g.h2 + kSplitCols - 1) / kSplitCols),
-                           dim3(kSplitThreads), split_col_lds(h), stream, g, h->py, ln.L1, g.p[0], g.pp,
+        L(k_cols_fft<-1>, dim3(nb * g.T * g.p[0], (g.h2 + kSplitCols - 1) / kSplitCols),
+                           dim3(kSplitThreads), split_col_lds(h), g, h->py, ln.L1, g.p[0], g.pp,
                            g.lo[0] - g.pl, g.lo[1], g.lo[1] + g.p[1]);
     } else if (f)
-      hipLaunchKernelGGL(f->lig, dim3((g.p[0] + 15) / 16 * 16, nb), dim3(kPlaneThreads), plane_lds(h), stream, g,
+      L(f->lig, dim3((g.p[0] + 15) / 16 * 16, nb), dim3(kPlaneThreads), plane_lds(h), g,
                          h->tw_y, h->tw_z, h->tw2, ln.cells, h->n_lig, h->w32, ln.L1);
     else
-      hipLaunchKernelGGL(k_lig_plane_fwd, dim3(g.p[0], nb), dim3(kPlaneThreads), plane_lds(h), stream,
+      L(k_lig_plane_fwd, dim3(g.p[0], nb), dim3(kPlaneThreads), plane_lds(h),
                          g, h->pz, h->py, h->tw2, ln.cells, h->n_lig, h->w32, ln.L1);
   }
   {
@@ -499,87 +546,131 @@ pd_status launch_fft_path(pd_scan* h, const pd_scan::Lane& ln, int nb, const dou
       const int mz = (g.pp + fc->lanes - 1) / fc->lanes;  // positions per lane that hold ligand planes
       int v = 0;
       while (v < 15 && fc->mz[v] < mz) ++v;
-      hipLaunchKernelGGL(fc->col[v], dim3(nblk), dim3(kColThreads), 0, stream, g, h->tw_x, ln.L1, h->Rlanes,
+      L(fc->col[v], dim3(nblk), dim3(kColThreads), 0, g, h->tw_x, ln.L1, h->Rlanes,
                          ln.S1, bpb, nb, nyb);
     } else
-      hipLaunchKernelGGL(k_col, dim3(ntile, nyb), dim3(kColThreads), col_lds(h), stream, g, h->px,
+      L(k_col, dim3(ntile, nyb), dim3(kColThreads), col_lds(h), g, h->px,
                          ln.L1, h->Rhat, ln.S1, h->C, bpb, nb);
   }
   {
     Timer tm(h, 3, stream);
     if (h->split && h->inv_fused) {
-      hipLaunchKernelGGL(kInvSplit, dim3(nb, g.n0), dim3(1024), kInvSplitLds, stream, g, h->tw_y, h->tw_z, h->tw2,
+      L(kInvSplit, dim3(nb, g.n0), dim3(1024), kInvSplitLds, g, h->tw_y, h->tw_z, h->tw2,
                          ln.S1, scale, h->K, ln.cval, ln.coff, grid_out, ln.thr);
     } else if (h->split) {
       if (h->fycol)
-        hipLaunchKernelGGL(h->fycol->inv, dim3(nb * g.n0, (g.h2 + h->fycol->cols - 1) / h->fycol->cols), dim3(256),
-                           h->fycol->lds, stream, g, h->tw_y, ln.S1, g.n0, g.n0, 0, 0, g.n1);
+        L(h->fycol->inv, dim3(nb * g.n0, (g.h2 + h->fycol->cols - 1) / h->fycol->cols), dim3(256),
+                           h->fycol->lds, g, h->tw_y, ln.S1, g.n0, g.n0, 0, 0, g.n1);
       else
-        hipLaunchKernelGGL(k_cols_fft<+1>, dim3(nb * g.n0, (g.h2 + kSplitCols - 1) / kSplitCols),
-                           dim3(kSplitThreads), split_col_lds(h), stream, g, h->py, ln.S1, g.n0, g.n0, 0, 0, g.n1);
+        L(k_cols_fft<+1>, dim3(nb * g.n0, (g.h2 + kSplitCols - 1) / kSplitCols),
+                           dim3(kSplitThreads), split_col_lds(h), g, h->py, ln.S1, g.n0, g.n0, 0, 0, g.n1);
       if (h->frows)
-        hipLaunchKernelGGL(h->frows->fn, dim3(nb, g.n0), dim3(512), h->frows->lds, stream, g, h->tw_z, h->tw2, ln.S1,
+        L(h->frows->fn, dim3(nb, g.n0), dim3(512), h->frows->lds, g, h->tw_z, h->tw2, ln.S1,
                            scale, h->K, ln.cval, ln.coff, grid_out, ln.thr);
       else
-        hipLaunchKernelGGL(k_inv_rows_topk, dim3(g.n0, nb), dim3(kSplitThreads), split_row_lds(h), stream, g,
+        L(k_inv_rows_topk, dim3(g.n0, nb), dim3(kSplitThreads), split_row_lds(h), g,
                            h->pz, h->tw2, ln.S1, scale, h->K, ln.cval, ln.coff, grid_out);
     } else if (f)
-      hipLaunchKernelGGL(f->inv, dim3(nb, g.n0), dim3(kPlaneThreads), plane_lds(h), stream, g, h->tw_y,
+      L(f->inv, dim3(nb, g.n0), dim3(kPlaneThreads), plane_lds(h), g, h->tw_y,
                          h->tw_z, h->tw2, ln.S1, scale, h->K, ln.cval, ln.coff, grid_out, ln.thr);
     else
-      hipLaunchKernelGGL(k_plane_inv_topk, dim3(g.n0, nb), dim3(kPlaneThreads), plane_lds(h), stream,
+      L(k_plane_inv_topk, dim3(g.n0, nb), dim3(kPlaneThreads), plane_lds(h),
                          g, h->pz, h->py, h->tw2, ln.S1, scale, h->K, ln.cval, ln.coff, grid_out);
   }
   PD_HIP(hipGetLastError());
   return PD_OK;
 }
 
-pd_status run_batches(pd_scan* h, int n_rot, const double* d_rots, int rot0, PeakOut* d_peaks,
-                      int* d_nout) {
+// One batch of nb rotations starting at rotation b0 of the call: the FFT path
+// and the per-rotation merge into the call's outputs.
+pd_status launch_batch(pd_scan* h, const pd_scan::Lane& ln, int b0, int nb, const double* d_rots, int rot0,
+                       PeakOut* d_peaks, int* d_nout, Launcher& L) {
+  const ScanGeom& g = h->g;
+  pd_status st = launch_fft_path(h, ln, nb, d_rots + 9 * (size_t)b0, nullptr, L);
+  if (st != PD_OK) return st;
+  {
+    Timer tm(h, 4, L.s);
+    if (g.n0 <= 64 * kMergePlanes)
+      L(k_merge_wave, dim3(nb), dim3(256), (size_t)g.n0 * h->K * 8, g, h->K, ln.cval, ln.coff,
+        h->d.max_val_corr, rot0 + b0, d_peaks + (size_t)b0 * h->K, d_nout + b0);
+    else
+      L(k_merge, dim3(nb), dim3(256), (size_t)g.n0 * h->K * 8, g, h->K, ln.cval, ln.coff, h->d.max_val_corr,
+        rot0 + b0, d_peaks + (size_t)b0 * h->K, d_nout + b0);
+  }
+  if (L.err != hipSuccess) return fail(PD_ERR_DEVICE, std::string("graph node: ") + hipGetErrorString(L.err));
+  PD_HIP(hipGetLastError());
+  return PD_OK;
+}
+
+pd_status check_run(pd_scan* h, int n_rot) {
   if (!h->receptor_ready) return fail(PD_ERR_STATE, "receptor not set");
   if (!h->ligand_ready) return fail(PD_ERR_STATE, "ligand not set");
   if (n_rot < 0) return fail(PD_ERR_INVALID_ARGUMENT, "n_rot < 0");
-  const ScanGeom& g = h->g;
-  // Batches alternate between the lanes (one lane while kernels are timed, so
-  // that each recorded duration belongs to one kernel alone); the lanes fork
-  // from and join back into the handle's stream.
-  // Under graph capture the batches stay on the handle's stream (the two-lane
-  // fork/join is not used there).
+  return PD_OK;
+}
+
+// Lanes a scan of nbatch batches uses: one while kernels are timed (so that
+// each recorded duration belongs to one kernel alone), with PD_SCAN_ONE_LANE,
+// or for a single batch.
+int lanes_for(const pd_scan* h, int nbatch) {
+  const bool single = h->timing || nbatch <= 1 || (h->d.flags & PD_SCAN_ONE_LANE) != 0;
+  return single ? 1 : pd_scan::kLanes;
+}
+
+// Eager scan on the handle's stream: batches go round the lanes' streams,
+// which fork from and join back into the handle's stream.
+pd_status run_batches(pd_scan* h, int n_rot, const double* d_rots, int rot0, PeakOut* d_peaks,
+                      int* d_nout) {
+  pd_status st = check_run(h, n_rot);
+  if (st != PD_OK) return st;
   const int nbatch = (n_rot + h->B - 1) / h->B;
   if (nbatch == 0) return PD_OK;
-  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-  PD_HIP(hipStreamIsCapturing(h->stream, &cs));
-  const bool single = h->timing || cs != hipStreamCaptureStatusNone || nbatch == 1 ||
-                      (h->d.flags & PD_SCAN_ONE_LANE) != 0;
-  const int nl = single ? 1 : pd_scan::kLanes;
+  const int nl = lanes_for(h, nbatch);
   pd_scan::Lane own = h->lane[0];
   own.s = h->stream;
-  if (!single) {
+  if (nl > 1) {
     PD_HIP(hipEventRecord(h->fork, h->stream));
     for (int l = 0; l < nl; ++l) PD_HIP(hipStreamWaitEvent(h->lane[l].s, h->fork, 0));
   }
   for (int i = 0; i < nbatch; ++i) {
+    const pd_scan::Lane& ln = nl == 1 ? own : h->lane[i % nl];
+    Launcher L;
+    L.s = ln.s;
     const int b0 = i * h->B;
-    const int nb = std::min(h->B, n_rot - b0);
-    const pd_scan::Lane& ln = single ? own : h->lane[i % nl];
-    pd_status st = launch_fft_path(h, ln, nb, d_rots + 9 * (size_t)b0, nullptr);
+    st = launch_batch(h, ln, b0, std::min(h->B, n_rot - b0), d_rots, rot0, d_peaks, d_nout, L);
     if (st != PD_OK) return st;
-    {
-      Timer tm(h, 4, ln.s);
-      if (g.n0 <= 64 * kMergePlanes)
-        hipLaunchKernelGGL(k_merge_wave, dim3(nb), dim3(256), (size_t)g.n0 * h->K * 8, ln.s, g, h->K, ln.cval,
-                           ln.coff, h->d.max_val_corr, rot0 + b0, d_peaks + (size_t)b0 * h->K, d_nout + b0);
-      else
-        hipLaunchKernelGGL(k_merge, dim3(nb), dim3(256), (size_t)g.n0 * h->K * 8, ln.s, g, h->K, ln.cval,
-                           ln.coff, h->d.max_val_corr, rot0 + b0, d_peaks + (size_t)b0 * h->K, d_nout + b0);
-    }
-    PD_HIP(hipGetLastError());
   }
-  if (!single)
+  if (nl > 1)
     for (int l = 0; l < nl; ++l) {
       PD_HIP(hipEventRecord(h->lane[l].done, h->lane[l].s));
       PD_HIP(hipStreamWaitEvent(h->stream, h->lane[l].done, 0));
     }
+  return PD_OK;
+}
+
+// The same scan as kernel nodes of a graph: batch i on lane i % nl waits for
+// the previous batch of its lane (which used the same buffers), the first batch
+// of each lane for `start`.  `tail` receives the last node of every lane.
+pd_status graph_batches(pd_scan* h, hipGraph_t graph, const std::vector<hipGraphNode_t>& start, int n_rot,
+                        const double* d_rots, PeakOut* d_peaks, int* d_nout, std::vector<hipGraphNode_t>& tail) {
+  pd_status st = check_run(h, n_rot);
+  if (st != PD_OK) return st;
+  const int nbatch = (n_rot + h->B - 1) / h->B;
+  const int nl = lanes_for(h, nbatch);
+  std::vector<std::vector<hipGraphNode_t>> last(nl, start);
+  for (int i = 0; i < nbatch; ++i) {
+    Launcher L;
+    L.graph = graph;
+    L.deps = last[i % nl];
+    const int b0 = i * h->B;
+    st = launch_batch(h, h->lane[i % nl], b0, std::min(h->B, n_rot - b0), d_rots, 0, d_peaks, d_nout, L);
+    if (st != PD_OK) return st;
+    last[i % nl] = L.deps;
+  }
+  tail.clear();
+  for (int l = 0; l < nl; ++l)
+    if (nl == 1 || l < nbatch) tail.insert(tail.end(), last[l].begin(), last[l].end());
+  if (tail.empty()) tail = start;
   return PD_OK;
 }
 
@@ -656,6 +747,45 @@ pd_status pd_ljcomp_params(int n_rec, const double* eps_r, const double* sigma_r
     w_rep[j] = e * (t3 * t3);
     w_att[j] = e * t3;
   }
+  return PD_OK;
+}
+
+pd_status pd_ljavg_params(int n_rec, const double* eps_r, const double* sigma_r, const double* alpha_r,
+                          int n_lig, const double* eps_l, const double* sigma_l, int mix, double cutoff,
+                          double* params, double* w) {
+  if (n_rec < 0 || n_lig <= 0) return fail(PD_ERR_INVALID_ARGUMENT, "atom counts");
+  if (!(cutoff > 0)) return fail(PD_ERR_INVALID_ARGUMENT, "cutoff must be > 0");
+  // ligand averages (blitz::mean, docking_molforce.hpp:395-396)
+  double se = 0, ss = 0;
+  for (int j = 0; j < n_lig; ++j) {
+    se += eps_l[j];
+    ss += sigma_l[j];
+  }
+  const double eps_avg = se / n_lig, sig_avg = ss / n_lig;
+  const double cut2 = cutoff * cutoff;
+  for (int a = 0; a < n_rec; ++a) {
+    // mixed parameters of atom a with the average ligand atom (:406-421)
+    const double sigma = mix == 0 ? (sigma_r[a] + sig_avg) / 2. : std::sqrt(sigma_r[a] * sig_avg);
+    const double eps = std::sqrt(eps_r[a] * eps_avg);
+    // SoftCoreLJ with the shifted-force cutoff (potentials_inc.hpp:185-208)
+    const double t = sigma * sigma * sigma, s6 = t * t;
+    const double as6 = alpha_r[a] * s6, e4s6 = 4. * eps * s6, e4s12 = e4s6 * s6;
+    // force at the cutoff, g2(cut^2, (cut, 0, 0)) before any shift (:221-228)
+    const double r6 = cut2 * cut2 * cut2, A = r6 + as6, Bq = s6 / (r6 + as6) - 1.;
+    const double Cq = -6. * (cut2 * cut2) * (e4s6 * A * Bq + e4s12) / (A * A * A);
+    const double g_cut = cutoff * Cq + cutoff * (0.0 / std::sqrt(cut2));
+    // energy at the cutoff, f2(cut^2) before any shift (:214-219)
+    const double f_cut = (e4s12 / A - e4s6) / A + 0.0 + 0.0 * std::sqrt(cut2);
+    const double gshift = -g_cut;
+    const double fshift = -f_cut - gshift * cutoff;
+    double* p = params + 5 * (size_t)a;
+    p[0] = e4s12;
+    p[1] = e4s6;
+    p[2] = as6;
+    p[3] = fshift;
+    p[4] = gshift;
+  }
+  for (int j = 0; j < n_lig; ++j) w[j] = 1;  // (:439-446)
   return PD_OK;
 }
 
@@ -818,8 +948,9 @@ pd_status pd_scan_create(const pd_scan_desc* desc, pd_scan** out) {
   return PD_OK;
 }
 
-void pd_scan_destroy(pd_scan* h) {
-  if (!h) return;
+pd_status pd_scan_destroy(pd_scan* h) {
+  if (!h) return PD_OK;
+  if (h->graph) return fail(PD_ERR_STATE, "the scan belongs to a graph: destroy the graph first");
   if (h->stream) hipStreamSynchronize(h->stream);
   for (auto& p : h->pending) { hipEventDestroy(p.a); hipEventDestroy(p.b); }
   free_batch(h);
@@ -834,6 +965,7 @@ void pd_scan_destroy(pd_scan* h) {
   if (h->fork) hipEventDestroy(h->fork);
   if (h->stream) hipStreamDestroy(h->stream);
   delete h;
+  return PD_OK;
 }
 
 pd_status pd_scan_project_receptor(pd_scan* h, int n_atoms, const double* pos, const pd_term* terms) {
@@ -903,6 +1035,7 @@ pd_status pd_scan_get_receptor_grids(pd_scan* h, double* grids) {
 
 pd_status pd_scan_set_ligand(pd_scan* h, int n_atoms, const double* pos, const double* weights) {
   if (!h || !pos || !weights || n_atoms < 1) return fail(PD_ERR_INVALID_ARGUMENT, "bad ligand arguments");
+  if (h->graph) return fail(PD_ERR_STATE, "the scan belongs to a graph: its buffers cannot change");
   PD_HIP(hipSetDevice(h->d.device));
   ScanGeom& g = h->g;
   // Rotation-invariant box of the ligand: every rotated atom lies within
@@ -970,6 +1103,7 @@ pd_status pd_scan_set_ligand(pd_scan* h, int n_atoms, const double* pos, const d
 pd_status pd_scan_set_batch(pd_scan* h, int batch) {
   if (!h) return fail(PD_ERR_INVALID_ARGUMENT, "null handle");
   if (batch < 1 || batch > 4096) return fail(PD_ERR_INVALID_ARGUMENT, "batch must be 1..4096");
+  if (h->graph) return fail(PD_ERR_STATE, "the scan belongs to a graph: its buffers cannot change");
   PD_HIP(hipSetDevice(h->d.device));
   PD_HIP(hipStreamSynchronize(h->stream));
   for (auto& ln : h->lane) PD_HIP(hipStreamSynchronize(ln.s));
@@ -1056,7 +1190,9 @@ pd_status pd_scan_correlation_grid(pd_scan* h, const double* rot, float* out) {
   PD_HIP(hipMemcpy(h->d_rots, rot, sizeof(double) * 9, hipMemcpyHostToDevice));
   pd_scan::Lane ln = h->lane[0];
   ln.s = h->stream;
-  st = launch_fft_path(h, ln, 1, h->d_rots, dgrid);
+  Launcher L;
+  L.s = h->stream;
+  st = launch_fft_path(h, ln, 1, h->d_rots, dgrid, L);
   if (st != PD_OK) return st;
   PD_HIP(hipMemcpyAsync(out, dgrid, sizeof(float) * h->N, hipMemcpyDeviceToHost, h->stream));
   PD_HIP(hipStreamSynchronize(h->stream));
@@ -1087,7 +1223,10 @@ void* pd_scan_stream(pd_scan* h) { return h ? (void*)h->stream : nullptr; }
 }  // extern "C"
 
 // ---------------------------------------------------------------------------
-// Several scans in one captured graph
+// Several scans in one graph.  The graph is built node by node (no stream
+// capture): each scan's batches become kernel nodes with explicit edges, as in
+// graph_batches; pairs on one graph lane follow each other, lanes are
+// independent branches from one root node.
 // ---------------------------------------------------------------------------
 struct pd_graph {
   int device = 0;
@@ -1095,15 +1234,15 @@ struct pd_graph {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   std::vector<pd_scan*> scans;
-  std::vector<hipEvent_t> events;
 };
 
 namespace {
 void graph_free(pd_graph* g) {
   if (!g) return;
+  for (pd_scan* h : g->scans)
+    if (h && h->graph == g) h->graph = nullptr;
   if (g->exec) hipGraphExecDestroy(g->exec);
   if (g->graph) hipGraphDestroy(g->graph);
-  for (hipEvent_t e : g->events) hipEventDestroy(e);
   if (g->origin) hipStreamDestroy(g->origin);
   delete g;
 }
@@ -1117,12 +1256,15 @@ pd_status pd_graph_create(int n, pd_scan* const* scans, const int* n_rot, const 
   if (!out || n < 1 || !scans || !n_rot || !d_rots || !d_peaks || !d_n_out || !lane || n_lanes < 1)
     return fail(PD_ERR_INVALID_ARGUMENT, "bad graph arguments");
   *out = nullptr;
+  for (int i = 0; i < n; ++i)
+    if (!scans[i]) return fail(PD_ERR_INVALID_ARGUMENT, "null scan handle");
   const int dev = scans[0]->d.device;
   for (int i = 0; i < n; ++i) {
     pd_scan* h = scans[i];
-    if (!h || h->d.device != dev) return fail(PD_ERR_INVALID_ARGUMENT, "scans must share one device");
+    if (h->d.device != dev) return fail(PD_ERR_INVALID_ARGUMENT, "scans must share one device");
+    if (h->graph) return fail(PD_ERR_STATE, "a scan already belongs to a graph");
     if (!h->receptor_ready || !h->ligand_ready) return fail(PD_ERR_STATE, "a scan has no receptor or ligand");
-    if (h->timing) return fail(PD_ERR_STATE, "kernel timing must be off while a graph is captured");
+    if (h->timing) return fail(PD_ERR_STATE, "kernel timing must be off while a graph holds the scan");
     if (lane[i] < 0 || lane[i] >= n_lanes) return fail(PD_ERR_INVALID_ARGUMENT, "lane out of range");
     if (n_rot[i] < 1 || !d_rots[i] || !d_peaks[i] || !d_n_out[i]) return fail(PD_ERR_INVALID_ARGUMENT, "bad pair");
     for (int j = 0; j < i; ++j)
@@ -1131,7 +1273,6 @@ pd_status pd_graph_create(int n, pd_scan* const* scans, const int* n_rot, const 
   PD_HIP(hipSetDevice(dev));
   pd_graph* g = new pd_graph();
   g->device = dev;
-  g->scans.assign(scans, scans + n);
   auto bad = [&](pd_status s) {
     graph_free(g);
     return s;
@@ -1140,36 +1281,22 @@ pd_status pd_graph_create(int n, pd_scan* const* scans, const int* n_rot, const 
     return bad(fail(PD_ERR_DEVICE, "hipStreamCreate failed"));
   for (int i = 0; i < n; ++i)
     if (hipStreamSynchronize(scans[i]->stream) != hipSuccess) return bad(fail(PD_ERR_DEVICE, "sync failed"));
-  g->events.resize(n + 1);
-  for (auto& e : g->events)
-    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
-      return bad(fail(PD_ERR_DEVICE, "hipEventCreate failed"));
-  // Fork: every lane starts from the origin; pairs of a lane run in the given
-  // order, each on its own scan's stream; the origin joins the last of each lane.
-  if (hipStreamBeginCapture(g->origin, hipStreamCaptureModeThreadLocal) != hipSuccess)
-    return bad(fail(PD_ERR_DEVICE, "hipStreamBeginCapture failed"));
-  pd_status st = PD_OK;
-  std::vector<hipEvent_t> last(n_lanes, g->events[n]);
-  if (hipEventRecord(g->events[n], g->origin) != hipSuccess) st = fail(PD_ERR_DEVICE, "hipEventRecord failed");
-  for (int i = 0; i < n && st == PD_OK; ++i) {
-    pd_scan* h = scans[i];
-    if (hipStreamWaitEvent(h->stream, last[lane[i]], 0) != hipSuccess) {
-      st = fail(PD_ERR_DEVICE, "hipStreamWaitEvent failed");
-      break;
-    }
-    st = run_batches(h, n_rot[i], d_rots[i], 0, reinterpret_cast<PeakOut*>(d_peaks[i]), d_n_out[i]);
-    if (st != PD_OK) break;
-    if (hipEventRecord(g->events[i], h->stream) != hipSuccess) st = fail(PD_ERR_DEVICE, "hipEventRecord failed");
-    last[lane[i]] = g->events[i];
+  if (hipGraphCreate(&g->graph, 0) != hipSuccess) return bad(fail(PD_ERR_DEVICE, "hipGraphCreate failed"));
+  hipGraphNode_t root = nullptr;
+  if (hipGraphAddEmptyNode(&root, g->graph, nullptr, 0) != hipSuccess)
+    return bad(fail(PD_ERR_DEVICE, "hipGraphAddEmptyNode failed"));
+  std::vector<std::vector<hipGraphNode_t>> lane_tail(n_lanes, std::vector<hipGraphNode_t>(1, root));
+  for (int i = 0; i < n; ++i) {
+    std::vector<hipGraphNode_t> tail;
+    pd_status st = graph_batches(scans[i], g->graph, lane_tail[lane[i]], n_rot[i], d_rots[i],
+                                 reinterpret_cast<PeakOut*>(d_peaks[i]), d_n_out[i], tail);
+    if (st != PD_OK) return bad(st);
+    lane_tail[lane[i]] = tail;
   }
-  for (int l = 0; l < n_lanes && st == PD_OK; ++l)
-    if (last[l] != g->events[n] && hipStreamWaitEvent(g->origin, last[l], 0) != hipSuccess)
-      st = fail(PD_ERR_DEVICE, "hipStreamWaitEvent failed");
-  const hipError_t ec = hipStreamEndCapture(g->origin, &g->graph);
-  if (st != PD_OK) return bad(st);
-  if (ec != hipSuccess) return bad(fail(PD_ERR_DEVICE, std::string("capture failed: ") + hipGetErrorString(ec)));
   if (hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0) != hipSuccess)
     return bad(fail(PD_ERR_DEVICE, "hipGraphInstantiate failed"));
+  g->scans.assign(scans, scans + n);
+  for (pd_scan* h : g->scans) h->graph = g;
   *out = g;
   return PD_OK;
 }

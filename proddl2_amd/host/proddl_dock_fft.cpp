@@ -406,12 +406,13 @@ Frame prepare_frame(MolForceParams& p, const Options& opt) {
   return f;
 }
 
-// FFTCorrelator::init grid size (docking_fft.hpp:541-553)
+// FFTCorrelator::init grid size (docking_fft.hpp:541-553): the findBestSize
+// edge as it is, odd edges included.
 int grid_edge(double lo, double hi, double h) {
   const int start = (int)((hi - (lo - h)) / h) - 1;
   const int n = pd_find_best_size(start);
   if (n < 0) throw Error("no FFT size");
-  return n + (n % 2);  // the GPU path needs an even last edge; extra padding is harmless
+  return n;
 }
 
 void check(pd_status s) {
@@ -422,9 +423,16 @@ void check(pd_status s) {
 // rot-scan task: Worker::init + Worker::run
 // ---------------------------------------------------------------------------
 void rot_scan(const Options& opt, MolForceParams& mfp) {
+  // App::init (docking_app.hpp:56-67): ljComp (two terms) or ljAvg (one term)
   const std::string pot = opt.str("potentialName", "ljComp");
-  if (pot != "ljComp") throw Error("not_supported_error: Unknown 'potentialName' parameter value: " + pot);
-  if (mfp.mix != 1) throw Error("ljComp requires mix == 1 (LJ_MIX_1)");
+  if (pot != "ljComp" && pot != "ljAvg")
+    throw Error("not_supported_error: Unknown 'potentialName' parameter value: " + pot);
+  if (pot == "ljComp" && mfp.mix != 1) throw Error("ljComp requires mix == 1 (LJ_MIX_1)");
+  // Worker::init post-filters (docking_app.hpp:369-398)
+  if (opt.flag("doClusterTranslations", false))
+    throw Error("not_supported_error: doClusterTranslations (TranClust) is not available in this build");
+  if (opt.integer("nMultimer", 0) > 1)
+    throw Error("not_supported_error: nMultimer > 1 (TranSymm) is not available in this build");
   const double h = opt.num("gridStep", 1.5);
   const double cut = opt.num("cutOffFft");
   const int maxNTrans = opt.integer("maxNTrans", 10);
@@ -443,7 +451,7 @@ void rot_scan(const Options& opt, MolForceParams& mfp) {
   pd_scan_desc d{};
   for (int a = 0; a < 3; ++a) d.n[a] = grid_edge(fr.box_lo[a], fr.box_hi[a], h);
   d.grid_step = h;
-  d.n_terms = 2;
+  d.n_terms = pot == "ljComp" ? 2 : 1;
   d.max_n_trans = maxNTrans;
   d.max_n_trans_inp = std::max(maxNTransInp, maxNTrans);
   d.max_val_corr = maxValCorr;
@@ -451,18 +459,29 @@ void rot_scan(const Options& opt, MolForceParams& mfp) {
   d.device = opt.integer("device", 0);
   pd_scan* h_ = nullptr;
   check(pd_scan_create(&d, &h_));
-  std::unique_ptr<pd_scan, void (*)(pd_scan*)> hs(h_, pd_scan_destroy);
+  std::unique_ptr<pd_scan, pd_status (*)(pd_scan*)> hs(h_, pd_scan_destroy);
 
   const Molecule& rec = mfp.mol[0];
   const Molecule& lig = mfp.mol[1];
-  std::vector<double> rep(2 * rec.n()), att(2 * rec.n()), wr(lig.n()), wa(lig.n());
-  check(pd_ljcomp_params(rec.n(), rec.eps.data(), rec.sigma.data(), rec.alpha.data(), lig.n(),
-                         lig.eps.data(), lig.sigma.data(), rep.data(), att.data(), wr.data(), wa.data()));
-  pd_term terms[2] = {{PD_FIELD_LJ_REP, cut, -1.0, rep.data()}, {PD_FIELD_LJ_ATT, cut, -1.0, att.data()}};
-  check(pd_scan_project_receptor(h_, rec.n(), rec.pos.data(), terms));
-  std::vector<double> w(2 * lig.n());
-  std::copy(wr.begin(), wr.end(), w.begin());
-  std::copy(wa.begin(), wa.end(), w.begin() + lig.n());
+  std::vector<double> w;  // n_terms x ligand atoms
+  if (pot == "ljComp") {
+    // MolForceLJComp = MolForceLJRep + MolForceLJAttr (docking_molforce.hpp:455-602)
+    std::vector<double> rep(2 * rec.n()), att(2 * rec.n()), wr(lig.n()), wa(lig.n());
+    check(pd_ljcomp_params(rec.n(), rec.eps.data(), rec.sigma.data(), rec.alpha.data(), lig.n(),
+                           lig.eps.data(), lig.sigma.data(), rep.data(), att.data(), wr.data(), wa.data()));
+    pd_term terms[2] = {{PD_FIELD_LJ_REP, cut, -1.0, rep.data()}, {PD_FIELD_LJ_ATT, cut, -1.0, att.data()}};
+    check(pd_scan_project_receptor(h_, rec.n(), rec.pos.data(), terms));
+    w.insert(w.end(), wr.begin(), wr.end());
+    w.insert(w.end(), wa.begin(), wa.end());
+  } else {
+    // MolForceLJ (docking_molforce.hpp:377-450)
+    std::vector<double> par(5 * rec.n()), wl(lig.n());
+    check(pd_ljavg_params(rec.n(), rec.eps.data(), rec.sigma.data(), rec.alpha.data(), lig.n(), lig.eps.data(),
+                          lig.sigma.data(), mfp.mix, cut, par.data(), wl.data()));
+    pd_term term = {PD_FIELD_LJ_FULL, cut, -1.0, par.data()};
+    check(pd_scan_project_receptor(h_, rec.n(), rec.pos.data(), &term));
+    w = wl;
+  }
   check(pd_scan_set_ligand(h_, lig.n(), lig.pos.data(), w.data()));
 
   const int nr = end - start;

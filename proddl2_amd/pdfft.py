@@ -35,7 +35,7 @@ EXPORTS = (
     "pd_scan_sync", "pd_scan_rescore64", "pd_scan_correlation_grid", "pd_scan_set_timing",
     "pd_scan_kernel_stats", "pd_scan_stream", "pd_scan_destroy", "pd_last_error",
     "pd_find_best_size", "pd_ljcomp_params", "pd_scan_set_batch", "pd_graph_create", "pd_graph_launch",
-    "pd_graph_sync", "pd_graph_stream", "pd_graph_destroy",
+    "pd_graph_sync", "pd_graph_stream", "pd_graph_destroy", "pd_ljavg_params",
 )
 
 KERNEL_NAMES = ("k_bin", "k_lig_plane_fwd", "k_col", "k_plane_inv_topk", "k_merge")
@@ -123,11 +123,11 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     lib.pd_scan_stream.argtypes = [P]
     lib.pd_scan_stream.restype = P
     lib.pd_scan_destroy.argtypes = [P]
-    lib.pd_scan_destroy.restype = None
     lib.pd_last_error.restype = C.c_char_p
     lib.pd_find_best_size.argtypes = [i]
     lib.pd_find_best_size.restype = i
     lib.pd_ljcomp_params.argtypes = [i, dp, dp, dp, i, dp, dp, dp, dp, dp, dp]
+    lib.pd_ljavg_params.argtypes = [i, dp, dp, dp, i, dp, dp, i, d, dp, dp]
     lib.pd_scan_set_batch.argtypes = [P, i]
     PP = C.POINTER(P)
     lib.pd_graph_create.argtypes = [i, PP, ip, PP, PP, PP, ip, i, PP]
@@ -138,7 +138,7 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     lib.pd_graph_destroy.argtypes = [P]
     lib.pd_graph_destroy.restype = None
     for name in EXPORTS:
-        if name not in ("pd_scan_destroy", "pd_last_error", "pd_find_best_size", "pd_scan_stream",
+        if name not in ("pd_last_error", "pd_find_best_size", "pd_scan_stream",
                         "pd_graph_stream", "pd_graph_destroy"):
             getattr(lib, name).restype = i
     _lib = lib
@@ -176,6 +176,19 @@ def ljcomp_params(eps_r, sigma_r, alpha_r, eps_l, sigma_l):
     return rep, att, wr, wa
 
 
+def ljavg_params(eps_r, sigma_r, alpha_r, eps_l, sigma_l, mix=1, cutoff=9.0):
+    """Per-atom ljAvg parameters (MolForceLJ, docking_molforce.hpp:377-450) via the C ABI:
+    (params[n_rec, 5] for FIELD_LJ_FULL, ligand weights[n_lig])."""
+    lib = load()
+    er, sr, ar = (np.ascontiguousarray(x, np.float64) for x in (eps_r, sigma_r, alpha_r))
+    el, sl = (np.ascontiguousarray(x, np.float64) for x in (eps_l, sigma_l))
+    par = np.empty((len(er), 5))
+    w = np.empty(len(el))
+    _check(lib.pd_ljavg_params(len(er), _dp(er), _dp(sr), _dp(ar), len(el), _dp(el), _dp(sl), int(mix),
+                               float(cutoff), _dp(par), _dp(w)))
+    return par, w
+
+
 class Scan:
     """One scan handle: receptor spectra resident on the GPU, rotations streamed."""
 
@@ -203,7 +216,7 @@ class Scan:
 
     def close(self) -> None:
         if getattr(self, "_h", None) is not None and self._h.value:
-            _lib.pd_scan_destroy(self._h)
+            _check(_lib.pd_scan_destroy(self._h))
             self._h = None
 
     def __del__(self):
@@ -308,10 +321,11 @@ class Scan:
 
 
 class Graph:
-    """Several scans captured into one HIP graph (pd_graph_*).
+    """Several scans in one HIP graph (pd_graph_*), built from kernel nodes.
 
     pairs: list of (Scan, n_rot, d_rots, d_peaks, d_n_out, lane) with device
-    pointers as ints; the buffers must stay alive as long as the graph."""
+    pointers as ints; the buffers must stay alive as long as the graph.  The
+    graph keeps its scans alive: a scan cannot be closed while a graph holds it."""
 
     def __init__(self, pairs, n_lanes: int):
         lib = load()
@@ -326,6 +340,7 @@ class Graph:
         g = P()
         _check(lib.pd_graph_create(n, scans, nrot, rots, peaks, nout, lane, int(n_lanes), C.byref(g)))
         self._g = g
+        self._scans = [p[0] for p in pairs]
 
     def launch(self) -> None:
         _check(_lib.pd_graph_launch(self._g))
@@ -337,6 +352,7 @@ class Graph:
         if getattr(self, "_g", None) is not None and self._g.value:
             _lib.pd_graph_destroy(self._g)
             self._g = None
+            self._scans = []
 
     def __del__(self):
         try:

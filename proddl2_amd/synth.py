@@ -183,6 +183,54 @@ def terms_for(cx: Complex, n_terms: int):
     raise ValueError("synthetic complexes define 2 or 4 terms")
 
 
+# Lennard-Jones parameters per atom type, in the units the reference's molforce
+# writer produces (python/lib/PRODDL/force_field_fft.py:91-99): sigma =
+# 2 * 2^(-1/6) * (r_min / 2) with r_min / 2 of 1.7-2.2 A, eps = 4.184 * |eps_charmm|
+# with |eps_charmm| of 0.02-0.2 kcal/mol; alpha is the scan's soft-core
+# parameter (default 0.4, config/noarch/proddl.json.in:11).
+LJ_SIGMA_RANGE = (3.03, 3.92)
+LJ_EPS_RANGE = (0.084, 0.84)
+LJ_CUTOFF = 9.0  # cutOffFft default (config/noarch/proddl.json.in:14)
+
+
+@dataclass
+class LJParams:
+    mass: np.ndarray   # (n,) per atom of receptor then ligand
+    eps: np.ndarray
+    sigma: np.ndarray
+    alpha: np.ndarray
+    n_rec: int
+
+
+def lj_params(cx: Complex, alpha=0.4, seed: int | None = None) -> LJParams:
+    """Per-atom mass, eps, sigma and alpha of receptor then ligand atoms, drawn per
+    atom type.  alpha: a number, or (lo, hi) for per-atom U(lo, hi) values."""
+    rng = np.random.default_rng(cx.seed + 7919 if seed is None else seed)
+    sig_t = rng.uniform(*LJ_SIGMA_RANGE, N_TYPES)
+    eps_t = rng.uniform(*LJ_EPS_RANGE, N_TYPES)
+    mass_t = rng.uniform(12.0, 32.0, N_TYPES)
+    types = np.concatenate([cx.receptor.types, cx.ligand.types])
+    n = types.shape[0]
+    if np.ndim(alpha) == 0:
+        al = np.full(n, float(alpha))
+    else:
+        al = rng.uniform(float(alpha[0]), float(alpha[1]), n)
+    return LJParams(mass_t[types], eps_t[types], sig_t[types], al, cx.receptor.pos.shape[0])
+
+
+def terms_ljcomp(cx: Complex, alpha=0.4, cutoff: float = LJ_CUTOFF, seed: int | None = None):
+    """The reference's own two-term potential, potentialName "ljComp"
+    (MolForceLJComp, docking_molforce.hpp:586-602): soft-core LJ repulsion and
+    attraction fields of the receptor (PD_FIELD_LJ_REP / LJ_ATT) with the ligand
+    weights sqrt(eps) sigma^6 and sqrt(eps) sigma^3, all from pd_ljcomp_params."""
+    from .pdfft import FIELD_LJ_ATT, FIELD_LJ_REP, ljcomp_params
+    p = lj_params(cx, alpha, seed)
+    nr = p.n_rec
+    rep, att, wr, wa = ljcomp_params(p.eps[:nr], p.sigma[:nr], p.alpha[:nr], p.eps[nr:], p.sigma[nr:])
+    rec_terms = [(FIELD_LJ_REP, cutoff, -1.0, rep), (FIELD_LJ_ATT, cutoff, -1.0, att)]
+    return rec_terms, np.stack([wr, wa])
+
+
 # Workloads, by grid and sizes.
 WORKLOADS = {
     "small-64": dict(n_rec=1800, n_lig=450, n=(64, 64, 64), h=1.2, n_terms=2, n_rot=500, K=10),
@@ -191,3 +239,15 @@ WORKLOADS = {
     "stress-256": dict(n_rec=12000, n_lig=3000, n=(256, 256, 256), h=1.0, n_terms=4,
                        n_rot=10000, K=10),
 }
+
+
+def terms_ljavg(cx: Complex, alpha=0.4, cutoff: float = LJ_CUTOFF, mix: int = 1, seed: int | None = None):
+    """The reference's second potential, potentialName "ljAvg" (MolForceLJ,
+    docking_molforce.hpp:377-450): one term, the full soft-core LJ with a
+    shifted-force cutoff of each receptor atom mixed with the ligand's average
+    atom (PD_FIELD_LJ_FULL), ligand weights 1; parameters from pd_ljavg_params."""
+    from .pdfft import FIELD_LJ_FULL, ljavg_params
+    p = lj_params(cx, alpha, seed)
+    nr = p.n_rec
+    par, w = ljavg_params(p.eps[:nr], p.sigma[:nr], p.alpha[:nr], p.eps[nr:], p.sigma[nr:], mix, cutoff)
+    return [(FIELD_LJ_FULL, cutoff, -1.0, par)], w.reshape(1, -1)

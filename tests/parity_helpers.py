@@ -1,0 +1,113 @@
+"""Shared bars and helpers of the GPU parity tests (libpdfft.so against the FP64 oracle).
+
+Bars:
+  * receptor projection (FP64 on the GPU): bit-identical to the oracle;
+  * correlation values (FP32 pipeline): |gpu - oracle| <= tol (see tolerance()); for
+    whole grids the larger of tol and 4 eps32 max|C| (see grid_tolerance());
+  * selected translations (integers): identical to the oracle's at every rank
+    whose oracle value is separated from its neighbours by more than 2*tol;
+  * FP64 rescoring: within 1e-9 relative of the oracle's direct sum.
+"""
+import numpy as np
+
+from proddl2_amd import pdfft, synth
+
+# FP32 error model of an FFT correlation C = IFFT(FFT(L) conj FFT(R)) / N:
+# a length-N FP32 FFT has relative RMS error ~ eps32 sqrt(log2 N), and the
+# correlation values of L and R have magnitude ~ ||L||_2 ||R||_2 / sqrt(N), so
+#   tol = 32 * eps32 * sqrt(log2 N) * sum_t ||w_t||_2 ||R_t||_2 / sqrt(N)
+# (the factor 32 covers the maximum over N cells and the three 1D passes).
+EPS32 = float(np.finfo(np.float32).eps)
+
+
+def tolerance(rec_grids, w, n):
+    N = int(np.prod(n))
+    s = sum(np.linalg.norm(w[t]) * np.linalg.norm(rec_grids[t]) for t in range(len(w)))
+    return 32.0 * EPS32 * np.sqrt(np.log2(N)) * s / np.sqrt(N)
+
+
+def grid_tolerance(tol, ref):
+    """Bar for a whole correlation grid: the norm-based tol, or 4 eps32 of the
+    largest |value|, whichever is larger.  The last passes round each value
+    relative to its own magnitude; on grids much larger than the molecules the
+    values are sparse, so ||L|| ||R|| / sqrt(N) underestimates the largest ones
+    (the cases here show max errors of 1.1 to 3.1 eps32 max|C|)."""
+    return max(tol, 4.0 * EPS32 * float(np.abs(ref).max()))
+
+
+def oracle_receptor(oracle, n, h, pos, rt):
+    return np.stack([oracle.project_receptor(n, h, pos, k, p, cut, md) for (k, cut, md, p) in rt])
+
+
+def build_case(oracle, n, h, n_rec, n_lig, n_terms, seed):
+    cx = synth.make_complex(n_rec, n_lig, seed)
+    rt, w = synth.terms_for(cx, n_terms)
+    return cx, rt, w, oracle_receptor(oracle, n, h, cx.receptor.pos, rt)
+
+
+def make_scan(n, h, T, K, cx, rt, w, batch=16, max_val=1e16, flags=0):
+    sc = pdfft.Scan(n, h, T, max_n_trans=K, max_val_corr=max_val, batch=batch, flags=flags)
+    sc.project_receptor(cx.receptor.pos, rt)
+    sc.set_ligand(cx.ligand.pos, w)
+    return sc
+
+
+def check_grids(oracle, sc, n, h, lig_pos, w, S, R, rots, bar="grid"):
+    """Correlation grids of the GPU against the oracle's for each rotation."""
+    tol = tolerance(R, w, n)
+    T = len(w)
+    worst = 0.0
+    for r in rots:
+        q = oracle.rotate_points(r, lig_pos)
+        L = np.stack([oracle.project_ligand(n, h, q, w[t]) for t in range(T)])
+        ref = oracle.correlate(S, L)
+        got = sc.correlation_grid(r).astype(np.float64)
+        err = np.abs(got - ref).max()
+        b = grid_tolerance(tol, ref) if bar == "grid" else tol
+        print(f"  max|err|={err:.3e} tol={tol:.3e} bar={b:.3e} range=[{ref.min():.3e},{ref.max():.3e}]",
+              flush=True)
+        assert err <= b
+        worst = max(worst, err / b)
+    return worst
+
+
+def compare_rotation_peaks(peaks, nout, val_o, off_o, s_o, tol, K, rot_index0=0, rot_ids=None):
+    """Per rotation: ranks, rotation indices, values within 3 tol, and offset and
+    displacement exact at every rank whose oracle value is more than 2 tol from
+    both neighbours (val_o holds K + 1 oracle values per rotation).  Returns the
+    number of ranks checked exactly."""
+    assert (nout == K).all()
+    checked = 0
+    for r in range(peaks.shape[0]):
+        v = val_o[r]
+        want_rot = rot_index0 + (r if rot_ids is None else rot_ids[r])
+        assert (peaks["rot"][r] == want_rot).all()
+        assert (peaks["rank"][r] == np.arange(K)).all()
+        for i in range(K):
+            lo_gap = v[i] - v[i - 1] if i > 0 else np.inf
+            hi_gap = v[i + 1] - v[i]
+            if lo_gap > 2 * tol and hi_gap > 2 * tol:
+                assert peaks["offset"][r, i] == off_o[r, i], (r, i, peaks[r], off_o[r], v)
+                assert (peaks["s"][r, i] == s_o[r, i]).all()
+                checked += 1
+            assert abs(peaks["value"][r, i] - v[i]) <= 3 * tol, (r, i, peaks["value"][r, i], v[i], tol)
+    return checked
+
+
+def check_peaks_terms(oracle, n, h, cx, rt, w, R, flags, n_rot, K=10, rot_seed=4, batch=5):
+    S = oracle.receptor_spectra(R)
+    rots = synth.rotation_set(n_rot, seed=rot_seed)
+    tol = tolerance(R, w, n)
+    nout_o, off_o, s_o, val_o = oracle.scan(n, h, S, cx.ligand.pos, w, rots, K=K + 1, max_val=1e16, n_threads=4)
+    with make_scan(n, h, len(w), K, cx, rt, w, batch=batch, flags=flags) as sc:
+        if n[0] >= 256:
+            G = sc.receptor_grids()
+            assert np.array_equal(G, R)
+        peaks, nout = sc.run(rots, rot_index0=100)
+        e64 = sc.rescore64(rots[peaks["rot"].reshape(-1) - 100], peaks["s"].reshape(-1, 3))
+    checked = compare_rotation_peaks(peaks, nout, val_o, off_o, s_o, tol, K, rot_index0=100)
+    # FP64 rescoring of every reported pose agrees with its FP32 value
+    assert np.abs(e64 - peaks["value"].reshape(-1)).max() <= 3 * tol
+    print(f"  checked {checked} of {len(rots) * K} ranks exactly (tol={tol:.3e})", flush=True)
+    assert checked >= len(rots) * K // 2
+    return checked

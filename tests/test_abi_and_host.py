@@ -97,3 +97,22 @@ def test_errors_map_to_status_codes():
     d.grid_step = -1.0
     assert lib.pd_scan_create(C.byref(d), C.byref(h)) == 1
     assert b"grid_step" in lib.pd_last_error()
+
+
+def test_graph_and_handle_argument_errors_without_device():
+    """Argument checks of the graph and handle entry points run before any device
+    call: a null scan handle in the list is refused (not dereferenced)."""
+    import ctypes as C
+    lib = pdfft.load()
+    P = C.c_void_p
+    scans = (P * 2)(None, None)
+    nrot = (C.c_int * 2)(4, 4)
+    ptrs = (P * 2)(1, 1)
+    lane = (C.c_int * 2)(0, 0)
+    g = P()
+    assert lib.pd_graph_create(2, scans, nrot, ptrs, ptrs, ptrs, lane, 1, C.byref(g)) == 1
+    assert b"null scan" in lib.pd_last_error()
+    assert lib.pd_graph_create(0, scans, nrot, ptrs, ptrs, ptrs, lane, 1, C.byref(g)) == 1
+    assert lib.pd_scan_destroy(None) == 0
+    assert lib.pd_scan_set_batch(None, 16) == 1
+    assert lib.pd_graph_launch(None) == 1

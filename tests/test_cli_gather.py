@@ -107,3 +107,30 @@ def test_gather_keeps_the_lowest_bounded(tmp_path):
     (val, _, _), scans, masses = _run(tmp_path, rec, lig, spin=1.5, max_matches=7, seed=5)
     allv = np.concatenate([pdoracle.read_scan_records(p)[3][:, :6].reshape(-1) for p in scans])
     assert np.array_equal(val, np.sort(allv[allv < 1e16])[:7].astype(np.float32))
+
+
+@pytest.mark.parametrize("option,value,what", [
+    ("doClusterTranslations", True, "TranClust"),
+    ("nMultimer", 2, "TranSymm"),
+    ("potentialName", "ljFoo", "potentialName"),
+])
+def test_rot_scan_refuses_what_it_cannot_do(tmp_path, option, value, what):
+    """Options the reference's Worker::init acts on (docking_app.hpp:56-67, 369-398)
+    that this build cannot honour end the task with not_supported_error before any
+    device work, instead of being ignored."""
+    if not os.path.exists(CLI):
+        pytest.skip("CLI not built")
+    rng = np.random.default_rng(4)
+    cx = synth.make_complex(60, 20, 4)
+    mf = str(tmp_path / "mf.pdmf")
+    _molforce(mf, cx.receptor.pos, cx.ligand.pos, rng)
+    ang = tmp_path / "ang.txt"
+    ang.write_text("1\n0 0 0\n")
+    opts = tmp_path / "opt.json"
+    opts.write_text(json.dumps({"gridStep": 1.5, "cutOffFft": 8.0, "anglesFile": str(ang), option: value}))
+    r = subprocess.run([CLI, "--task", "rot-scan", "--options", str(opts), "--molforce-params", mf,
+                        "--fft-rot-grid-start", "0", "--fft-rot-grid-end", "1", "--fft-rot-scan-res",
+                        str(tmp_path / "o.bin")], capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0
+    assert "not_supported_error" in r.stderr and what in r.stderr, r.stderr
+    assert not os.path.exists(tmp_path / "o.bin")

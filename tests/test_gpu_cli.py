@@ -1,6 +1,7 @@
 """The worker flow of the reference's Makeflow rules (python/lib/PRODDL/dock.py:86-113)
-through this build's proddl-dock-fft: two `rot-scan` tasks over rotation ranges,
-then `gather` over their outputs."""
+through this build's proddl-dock-fft: `rot-scan` tasks over rotation ranges, then
+`gather` over their outputs.  The rot-scan records are compared with the FP64
+oracle's scan of the same rotations in the same reference frame."""
 import json
 import os
 import subprocess
@@ -9,11 +10,95 @@ import numpy as np
 import pytest
 
 from oracle import foreman, pdoracle
-from proddl2_amd import mfp, synth
+from proddl2_amd import mfp, pdfft, synth
 from proddl2_amd.build import CLI
 
 pytestmark = pytest.mark.gpu
 torch = pytest.importorskip("torch")
+
+SPIN = 23.3734432  # startingSpin default (docking_app.hpp via docking_molstruct.hpp:248-266)
+
+
+def _inputs(tmp_path, n_rec=400, n_lig=90, seed=12, n_ang=40):
+    rng = np.random.default_rng(11)
+    cx = synth.make_complex(n_rec, n_lig, seed)
+    rec, lig = cx.receptor.pos + 7.0, cx.ligand.pos - 3.0  # arbitrary original frames
+    pos = np.concatenate([rec, lig])
+    n = len(pos)
+    p = {"mass": rng.uniform(10, 16, n), "eps": rng.uniform(0.1, 0.3, n), "sigma": rng.uniform(3.0, 4.0, n),
+         "alpha": rng.uniform(0.2, 0.5, n)}
+    mf = str(tmp_path / "mf.pdmf")
+    mfp.write(mf, {"pos": pos, **p, "mol_offsets": np.array([[0, len(rec)], [len(rec), n]], np.int32),
+                   "mix": np.array([1], np.int32)})
+    ang = tmp_path / "angles.txt"
+    a = rng.uniform(-180, 180, (n_ang, 3))
+    ang.write_text(f"{n_ang}\n" + "\n".join(f"{x:.17g} {y:.17g} {z:.17g}" for x, y, z in a) + "\n")
+    return rec, lig, p, mf, str(ang), a
+
+
+def _docking_frame(rec, lig, lig_mass, h, cut):
+    """The rot-scan frame (docking_molstruct.hpp:199-266 as the CLI states it):
+    receptor box centred at the origin, ligand spun about its centre of mass and
+    its diameter centred at the origin; grid edges from the padded box."""
+    tR, _ = foreman.frames(rec, lig, lig_mass, SPIN)
+    rec_d = rec + tR[1]
+    rs = pdoracle.euler_to_matrix([SPIN, SPIN * 3.7877, SPIN * -4.513432])
+    cm = (lig_mass[:, None] * lig).sum(0) / lig_mass.sum()
+    spun = (lig - cm) @ rs.T + cm
+    d2 = ((spun[:, None, :] - spun[None, :, :]) ** 2).sum(-1)
+    i, j = np.unravel_index(np.argmax(np.triu(d2, 1)), d2.shape)
+    lig_d = spun - 0.5 * (spun[i] + spun[j])
+    pad = 2 * h + np.sqrt(d2[i, j]) / 2 + cut
+    lo, hi = rec_d.min(0), rec_d.max(0)
+    n = tuple(pdoracle.grid_size_from_box(lo[d] - pad, hi[d] + pad, h) for d in range(3))
+    return rec_d, lig_d, n
+
+
+def _rot_scan(tmp_path, opts, mf, lo, hi):
+    out = str(tmp_path / f"scan_{lo}.bin")
+    print(f"rot-scan [{lo},{hi})", flush=True)
+    r = subprocess.run([CLI, "--task", "rot-scan", "--options", str(opts), "--molforce-params", mf,
+                        "--fft-rot-grid-start", str(lo), "--fft-rot-grid-end", str(hi), "--fft-rot-scan-res", out],
+                       capture_output=True, timeout=120, text=True)
+    assert r.returncode == 0, r.stderr
+    print("  " + r.stdout.strip(), flush=True)
+    return out
+
+
+def _check_against_oracle(oracle_terms, rec, lig, p, n_rec, h, cut, rots, tran, vals, K):
+    lig_mass = p["mass"][n_rec:]
+    rec_d, lig_d, n = _docking_frame(rec, lig, lig_mass, h, cut)
+    rt, w = oracle_terms(p, n_rec)
+    R = np.stack([pdoracle.project_receptor(n, h, rec_d, k, par, c, md) for (k, c, md, par) in rt])
+    S = pdoracle.receptor_spectra(R)
+    from tests.parity_helpers import tolerance
+    tol = tolerance(R, w, n)
+    nout_o, off_o, s_o, val_o = pdoracle.scan(n, h, S, lig_d, w, rots, K=K + 1, max_val=1e16, n_threads=4)
+    exact = 0
+    for r in range(len(rots)):
+        v = val_o[r]
+        for i in range(K):
+            lo_gap = v[i] - v[i - 1] if i else np.inf
+            if lo_gap > 2 * tol and v[i + 1] - v[i] > 2 * tol:
+                assert np.array_equal(tran[r, i], h * s_o[r, i]), (r, i, tran[r, i], s_o[r, i])
+                assert abs(vals[r, i] - v[i]) <= 1e-9 * max(1.0, abs(v[i])), (r, i, vals[r, i], v[i])
+                exact += 1
+            assert abs(vals[r, i] - v[i]) <= 3 * tol
+    print(f"  grid {n}: {exact} of {len(rots) * K} ranks exact against the oracle (tol={tol:.3e})", flush=True)
+    assert exact >= len(rots) * K // 2
+    return n
+
+
+def _ljcomp_terms(p, nr):
+    rep, att, wr, wa = pdoracle.ljcomp_params(p["eps"][:nr], p["sigma"][:nr], p["alpha"][:nr], p["eps"][nr:],
+                                             p["sigma"][nr:])
+    return [(pdfft.FIELD_LJ_REP, 8.0, -1.0, rep), (pdfft.FIELD_LJ_ATT, 8.0, -1.0, att)], np.stack([wr, wa])
+
+
+def _ljavg_terms(p, nr):
+    par, w = pdoracle.ljavg_params(p["eps"][:nr], p["sigma"][:nr], p["alpha"][:nr], p["eps"][nr:], p["sigma"][nr:],
+                                   1, 8.0)
+    return [(pdfft.FIELD_LJ_FULL, 8.0, -1.0, par)], w.reshape(1, -1)
 
 
 def test_rot_scan_ranges_then_gather(tmp_path):
@@ -21,30 +106,13 @@ def test_rot_scan_ranges_then_gather(tmp_path):
         pytest.skip("no GPU")
     if not os.path.exists(CLI):
         pytest.skip("CLI not built")
-    rng = np.random.default_rng(11)
-    cx = synth.make_complex(400, 90, 12)
-    rec, lig = cx.receptor.pos + 7.0, cx.ligand.pos - 3.0  # arbitrary original frames
-    pos = np.concatenate([rec, lig])
-    n = len(pos)
-    masses = rng.uniform(10, 16, n)
-    mf = str(tmp_path / "mf.pdmf")
-    mfp.write(mf, {"pos": pos, "mass": masses, "eps": rng.uniform(0.1, 0.3, n),
-                   "sigma": rng.uniform(3.0, 4.0, n), "alpha": rng.uniform(0.2, 0.5, n),
-                   "mol_offsets": np.array([[0, len(rec)], [len(rec), n]], np.int32),
-                   "mix": np.array([1], np.int32)})
-    ang = tmp_path / "angles.txt"
-    a = rng.uniform(-180, 180, (40, 3))
-    ang.write_text("40\n" + "\n".join(f"{x:.17g} {y:.17g} {z:.17g}" for x, y, z in a) + "\n")
+    rec, lig, p, mf, ang_path, a = _inputs(tmp_path)
     opts = tmp_path / "opt.json"
-    opts.write_text(json.dumps({"gridStep": 1.5, "cutOffFft": 8.0, "anglesFile": str(ang), "maxNTrans": 5,
+    opts.write_text(json.dumps({"gridStep": 1.5, "cutOffFft": 8.0, "anglesFile": ang_path, "maxNTrans": 5,
                                 "maxValCorr": 1e16, "maxNRigidMatches": 30}))
     outs = []
     for lo, hi in ((0, 25), (25, 40)):
-        out = str(tmp_path / f"scan_{lo}.bin")
-        print(f"rot-scan [{lo},{hi})", flush=True)
-        subprocess.run([CLI, "--task", "rot-scan", "--options", str(opts), "--molforce-params", mf,
-                        "--fft-rot-grid-start", str(lo), "--fft-rot-grid-end", str(hi), "--fft-rot-scan-res", out],
-                       check=True, capture_output=True, timeout=120)
+        out = _rot_scan(tmp_path, opts, mf, lo, hi)
         rots, counts, tran, vals = pdoracle.read_scan_records(out)
         assert len(rots) == hi - lo and (counts == 5).all()
         assert np.all(np.diff(vals[:, :5], axis=1) >= 0)
@@ -52,6 +120,7 @@ def test_rot_scan_ranges_then_gather(tmp_path):
         for r in range(len(rots)):
             want = pdoracle.euler_to_matrix(np.radians(a[lo + r]))
             assert np.abs(rots[r].reshape(3, 3) - want).max() < 1e-12
+        _check_against_oracle(_ljcomp_terms, rec, lig, p, len(rec), 1.5, 8.0, rots, tran[:, :5], vals[:, :5], 5)
         outs.append(out)
     lst = tmp_path / "list.txt"
     lst.write_text("\n".join(outs) + "\n")
@@ -59,9 +128,23 @@ def test_rot_scan_ranges_then_gather(tmp_path):
     subprocess.run([CLI, "--task", "gather", "--options", str(opts), "--molforce-params", mf,
                     "--fft-rot-scan-list", str(lst), "--fft-res", res], check=True, capture_output=True, timeout=120)
     val, ang_o, xyz = foreman.read_iorigid(res)
-    want = foreman.gather(outs, rec, lig, masses[len(rec):], 23.3734432, 30)
+    want = foreman.gather(outs, rec, lig, p["mass"][len(rec):], SPIN, 30)
     assert len(val) == 30
     for i, (v, m, t) in enumerate(want):
         assert val[i] == np.float32(v)
         assert np.abs(pdoracle.euler_to_matrix(ang_o[i].astype(np.float64)) - m).max() < 2e-6
         assert np.abs(xyz[i] - t).max() < 1e-4
+
+
+def test_rot_scan_ljavg_matches_oracle(tmp_path):
+    """potentialName ljAvg (App::init, docking_app.hpp:60-64): one LJ_FULL term."""
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    rec, lig, p, mf, ang_path, a = _inputs(tmp_path, seed=13, n_ang=12)
+    opts = tmp_path / "opt.json"
+    opts.write_text(json.dumps({"gridStep": 1.5, "cutOffFft": 8.0, "anglesFile": ang_path, "maxNTrans": 6,
+                                "maxValCorr": 1e16, "potentialName": "ljAvg"}))
+    out = _rot_scan(tmp_path, opts, mf, 0, 12)
+    rots, counts, tran, vals = pdoracle.read_scan_records(out)
+    assert len(rots) == 12 and (counts == 6).all()
+    _check_against_oracle(_ljavg_terms, rec, lig, p, len(rec), 1.5, 8.0, rots, tran[:, :6], vals[:, :6], 6)

@@ -1,0 +1,102 @@
+"""GPU parity of the reference's own potential, ljComp (soft-core LJ repulsion +
+attraction; MolForceLJComp, docking_molforce.hpp:455-602; fields
+potentials_inc.hpp:289-352), against the FP64 oracle.
+
+The parameters go through pd_ljcomp_params (the library's host code) and the
+receptor fields through PD_FIELD_LJ_REP / PD_FIELD_LJ_ATT on the GPU.  Bars as
+in parity_helpers: receptor grids bit-identical, correlation grids within the
+FP32 bar, peak integers exact outside the tolerance band.  The reference warns
+that FP32 FFTs are unstable for LJ at small soft-core alpha
+(docking_molforce.hpp:115-118): the small-alpha cases measure how the norm-based
+band holds there.
+"""
+import numpy as np
+import pytest
+
+from tests.conftest import gpu_available
+
+pytestmark = pytest.mark.gpu
+
+if not gpu_available():
+    pytest.skip("no GPU", allow_module_level=True)
+
+from proddl2_amd import synth  # noqa: E402
+
+from tests.parity_helpers import (  # noqa: E402
+    check_grids, check_peaks_terms, make_scan, oracle_receptor, tolerance)
+
+# grid, h, n_rec, n_lig, alpha (number: every atom; pair: per-atom U(lo, hi), as
+# force_field_fft.py writes one value per atom), cutOffFft, flags
+LJ_CASES = [
+    ((48, 48, 48), 1.5, 700, 220, (0.2, 0.5), 9.0, 0),       # edge-specialised kernels
+    ((64, 60, 72), 1.5, 900, 250, 0.4, 9.0, 0),              # generic runtime-plan kernels
+    ((64, 64, 64), 1.2, 1800, 450, (0.2, 0.5), 9.0, 0),      # the small plain-method configuration
+    ((48, 48, 48), 1.5, 700, 220, 0.05, 9.0, 0),             # small alpha
+    ((64, 64, 64), 1.2, 1800, 450, 0.02, 9.0, 1),            # smaller alpha, split plane kernels
+]
+
+
+def lj_case(oracle, n, h, n_rec, n_lig, alpha, cut, seed):
+    cx = synth.make_complex(n_rec, n_lig, seed)
+    rt, w = synth.terms_ljcomp(cx, alpha=alpha, cutoff=cut)
+    # the library's ljComp parameters equal the oracle's bit for bit
+    p = synth.lj_params(cx, alpha)
+    nr = p.n_rec
+    rep, att, wr, wa = oracle.ljcomp_params(p.eps[:nr], p.sigma[:nr], p.alpha[:nr], p.eps[nr:], p.sigma[nr:])
+    assert np.array_equal(rt[0][3], rep) and np.array_equal(rt[1][3], att)
+    assert np.array_equal(w[0], wr) and np.array_equal(w[1], wa)
+    R = oracle_receptor(oracle, n, h, cx.receptor.pos, rt)
+    return cx, rt, w, R
+
+
+@pytest.mark.parametrize("n,h,n_rec,n_lig,alpha,cut,flags", LJ_CASES)
+def test_ljcomp_receptor_grids_bit_identical(oracle, n, h, n_rec, n_lig, alpha, cut, flags):
+    print(f"case ljComp receptor n={n} alpha={alpha}", flush=True)
+    cx, rt, w, R = lj_case(oracle, n, h, n_rec, n_lig, alpha, cut, seed=21)
+    with make_scan(n, h, 2, 4, cx, rt, w, flags=flags) as sc:
+        G = sc.receptor_grids()
+    bad = np.argwhere(G != R)
+    assert bad.shape[0] == 0, f"{bad.shape[0]} cells differ, first {bad[:3]}"
+
+
+@pytest.mark.parametrize("n,h,n_rec,n_lig,alpha,cut,flags", LJ_CASES)
+def test_ljcomp_correlation_grid(oracle, n, h, n_rec, n_lig, alpha, cut, flags):
+    print(f"case ljComp correlation grid n={n} alpha={alpha} flags={flags}", flush=True)
+    cx, rt, w, R = lj_case(oracle, n, h, n_rec, n_lig, alpha, cut, seed=22)
+    S = oracle.receptor_spectra(R)
+    rots = synth.rotation_set(3, seed=7)
+    with make_scan(n, h, 2, 4, cx, rt, w, flags=flags) as sc:
+        worst = check_grids(oracle, sc, n, h, cx.ligand.pos, w, S, R, rots)
+    print(f"  worst error / bar = {worst:.3f}", flush=True)
+
+
+@pytest.mark.parametrize("n,h,n_rec,n_lig,alpha,cut,flags", LJ_CASES)
+def test_ljcomp_scan_peaks_match_oracle(oracle, n, h, n_rec, n_lig, alpha, cut, flags):
+    print(f"case ljComp peaks n={n} alpha={alpha} flags={flags}", flush=True)
+    cx, rt, w, R = lj_case(oracle, n, h, n_rec, n_lig, alpha, cut, seed=23)
+    check_peaks_terms(oracle, n, h, cx, rt, w, R, flags, n_rot=12, K=10, rot_seed=8)
+
+
+def test_ljcomp_two_copies_double_the_energies(oracle):
+    """MolForceLJCompTest (docking_molforce.hpp:605-626) on the GPU: the ljComp
+    pair given twice (four terms) returns the same poses with doubled energies."""
+    n, h = (48, 48, 48), 1.5
+    cx, rt, w, R = lj_case(oracle, n, h, 700, 220, 0.4, 9.0, seed=24)
+    rots = synth.rotation_set(6, seed=9)
+    with make_scan(n, h, 2, 10, cx, rt, w, batch=4) as sc:
+        p1, n1 = sc.run(rots)
+    with make_scan(n, h, 4, 10, cx, rt + rt, np.concatenate([w, w]), batch=4) as sc:
+        p2, n2 = sc.run(rots)
+    tol = tolerance(R, w, n)
+    assert (n1 == 10).all() and (n2 == 10).all()
+    assert np.all(np.abs(p2["value"] - 2 * p1["value"]) <= 4 * tol)
+    v = p1["value"].astype(np.float64)
+    same = 0
+    for r in range(len(rots)):
+        for i in range(10):
+            lo = v[r, i] - v[r, i - 1] if i else np.inf
+            hi = v[r, i + 1] - v[r, i] if i < 9 else np.inf
+            if lo > 2 * tol and hi > 2 * tol:
+                assert p1["offset"][r, i] == p2["offset"][r, i]
+                same += 1
+    assert same >= len(rots) * 10 // 2

@@ -20,41 +20,8 @@ if not gpu_available():  # collected on CPU too, but every test needs the device
 
 from proddl2_amd import pdfft, synth  # noqa: E402
 
-# FP32 error model of an FFT correlation C = IFFT(FFT(L) conj FFT(R)) / N:
-# a length-N FP32 FFT has relative RMS error ~ eps32 sqrt(log2 N), and the
-# correlation values of L and R have magnitude ~ ||L||_2 ||R||_2 / sqrt(N), so
-#   tol = 32 * eps32 * sqrt(log2 N) * sum_t ||w_t||_2 ||R_t||_2 / sqrt(N)
-# (the factor 32 covers the maximum over N cells and the three 1D passes).
-EPS32 = float(np.finfo(np.float32).eps)
-
-
-def tolerance(rec_grids, w, n):
-    N = int(np.prod(n))
-    s = sum(np.linalg.norm(w[t]) * np.linalg.norm(rec_grids[t]) for t in range(len(w)))
-    return 32.0 * EPS32 * np.sqrt(np.log2(N)) * s / np.sqrt(N)
-
-
-def grid_tolerance(tol, ref):
-    """Bar for a whole correlation grid: the norm-based tol, or 4 eps32 of the
-    largest |value|, whichever is larger.  The last passes round each value
-    relative to its own magnitude; on grids much larger than the molecules the
-    values are sparse, so ||L|| ||R|| / sqrt(N) underestimates the largest ones
-    (the cases here show max errors of 1.1 to 3.1 eps32 max|C|)."""
-    return max(tol, 4.0 * EPS32 * float(np.abs(ref).max()))
-
-
-def build_case(oracle, n, h, n_rec, n_lig, n_terms, seed):
-    cx = synth.make_complex(n_rec, n_lig, seed)
-    rt, w = synth.terms_for(cx, n_terms)
-    R = np.stack([oracle.project_receptor(n, h, cx.receptor.pos, k, p, cut, md) for (k, cut, md, p) in rt])
-    return cx, rt, w, R
-
-
-def make_scan(n, h, T, K, cx, rt, w, batch=16, max_val=1e16, flags=0):
-    sc = pdfft.Scan(n, h, T, max_n_trans=K, max_val_corr=max_val, batch=batch, flags=flags)
-    sc.project_receptor(cx.receptor.pos, rt)
-    sc.set_ligand(cx.ligand.pos, w)
-    return sc
+from tests.parity_helpers import (  # noqa: E402
+    EPS32, build_case, check_grids, check_peaks_terms, grid_tolerance, make_scan, tolerance)
 
 
 CASES = [
@@ -186,34 +153,7 @@ def test_scan_peaks_match_oracle(oracle, n, h, n_rec, n_lig, T, flags):
 
 def check_peaks(oracle, n, h, n_rec, n_lig, T, flags, n_rot, seed, K=10):
     cx, rt, w, R = build_case(oracle, n, h, n_rec, n_lig, T, seed=seed)
-    S = oracle.receptor_spectra(R)
-    rots = synth.rotation_set(n_rot, seed=4)
-    tol = tolerance(R, w, n)
-    nout_o, off_o, s_o, val_o = oracle.scan(n, h, S, cx.ligand.pos, w, rots, K=K + 1, max_val=1e16, n_threads=4)
-    with make_scan(n, h, T, K, cx, rt, w, batch=5, flags=flags) as sc:
-        if n[0] >= 256:
-            G = sc.receptor_grids()
-            assert np.array_equal(G, R)
-        peaks, nout = sc.run(rots, rot_index0=100)
-        e64 = sc.rescore64(rots[peaks["rot"].reshape(-1) - 100], peaks["s"].reshape(-1, 3))
-    assert (nout == K).all()
-    checked = 0
-    for r in range(len(rots)):
-        v = val_o[r]
-        assert (peaks["rot"][r] == 100 + r).all()
-        assert (peaks["rank"][r] == np.arange(K)).all()
-        for i in range(K):
-            lo_gap = v[i] - v[i - 1] if i > 0 else np.inf
-            hi_gap = v[i + 1] - v[i]
-            if lo_gap > 2 * tol and hi_gap > 2 * tol:
-                assert peaks["offset"][r, i] == off_o[r, i], (r, i, peaks[r], off_o[r], v)
-                assert (peaks["s"][r, i] == s_o[r, i]).all()
-                checked += 1
-            assert abs(peaks["value"][r, i] - v[i]) <= tol + 2 * tol
-    # FP64 rescoring of every reported pose agrees with its FP32 value
-    assert np.abs(e64 - peaks["value"].reshape(-1)).max() <= 3 * tol
-    print(f"  checked {checked} of {len(rots) * K} ranks exactly (tol={tol:.3e})", flush=True)
-    assert checked >= len(rots) * K // 2
+    return check_peaks_terms(oracle, n, h, cx, rt, w, R, flags, n_rot, K=K)
 
 
 def test_one_kernel_inverse_same_bits_as_two(oracle):
@@ -265,13 +205,24 @@ def test_max_val_corr_filter(oracle):
     S = oracle.receptor_spectra(R)
     rots = synth.rotation_set(4, seed=6)
     nout_o, off_o, s_o, val_o = oracle.scan(n, h, S, cx.ligand.pos, w, rots, K=10, max_val=1e16)
+    tol = tolerance(R, w, n)
     thr = float(np.median(val_o[:, 5]))
     with make_scan(n, h, T, 10, cx, rt, w, max_val=thr) as sc:
         peaks, nout = sc.run(rots)
+    exact = 0
     for r in range(len(rots)):
         assert (peaks["value"][r, : nout[r]] < thr).all()
         expect = int((val_o[r] < thr).sum())
-        assert abs(int(nout[r]) - expect) <= 1
+        # Only a value within the FP32 band of the threshold may fall on the
+        # other side of it on the GPU: the count may differ by at most the number
+        # of such values (the threshold is an oracle value, so each rotation
+        # whose 6th value set it has at least one).
+        ambiguous = int((np.abs(val_o[r] - thr) <= tol).sum())
+        assert abs(int(nout[r]) - expect) <= ambiguous, (r, nout[r], expect, ambiguous)
+        exact += ambiguous == 0
+        # the kept values are the oracle's first nout values, within the band
+        assert np.all(np.abs(peaks["value"][r, : nout[r]] - val_o[r, : nout[r]]) <= 3 * tol)
+    print(f"  {exact} of {len(rots)} rotations had no value within the band of maxValCorr", flush=True)
 
 
 def test_ligand_outside_grid_is_domain_error(oracle):

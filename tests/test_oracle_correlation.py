@@ -86,6 +86,62 @@ def test_ljcomp_library_matches_oracle(oracle):
         assert np.array_equal(x, y)
 
 
+def _softcore_lj(r, eps, sig, alpha):
+    """Soft-core LJ 4 eps (s12 / A^2 - s6 / A), A = r^6 + alpha s6, and dU/dr (analytic)."""
+    s6 = sig ** 6
+    A = r ** 6 + alpha * s6
+    u = 4 * eps * (s6 * s6 / A ** 2 - s6 / A)
+    du = 4 * eps * (-2 * s6 * s6 / A ** 3 + s6 / A ** 2) * 6 * r ** 5
+    return u, du
+
+
+@pytest.mark.parametrize("mix", [0, 1])
+def test_ljavg_field_closed_form_with_shifted_force(oracle, mix):
+    """ljAvg (MolForceLJ, docking_molforce.hpp:377-450; SoftCoreLJ with
+    CUT_POLICY_SHIFT_FORCE, potentials_inc.hpp:185-219): the receptor field of
+    atom a is U(r) - U(c) - U'(c) (r - c) for the soft-core LJ of atom a mixed
+    with the ligand's average atom; it and its slope vanish at the cutoff c."""
+    rng = np.random.default_rng(3)
+    eps_r, sig_r, al_r = rng.uniform(0.1, 0.8, 3), rng.uniform(3.0, 3.9, 3), rng.uniform(0.0, 0.6, 3)
+    eps_l, sig_l = rng.uniform(0.1, 0.8, 7), rng.uniform(3.0, 3.9, 7)
+    cut = 8.5
+    par, w = oracle.ljavg_params(eps_r, sig_r, al_r, eps_l, sig_l, mix, cut)
+    assert np.all(w == 1)
+    for a in range(3):
+        sig = (sig_r[a] + sig_l.mean()) / 2 if mix == 0 else np.sqrt(sig_r[a] * sig_l.mean())
+        eps = np.sqrt(eps_r[a] * eps_l.mean())
+        uc, duc = _softcore_lj(cut, eps, sig, al_r[a])
+        # single atom at a cell-centre offset: every cell within the cutoff
+        n, h = (26, 26, 26), 0.7
+        c = np.array([[0.11, -0.23, 0.05]])
+        g = oracle.project_receptor(n, h, c, pdfft.FIELD_LJ_FULL, par[a:a + 1], cut)
+        idx = np.stack(np.meshgrid(*[np.arange(k) for k in n], indexing="ij"), -1)
+        zsc = oracle.geometry_centered(h, 26)[1]
+        r2 = ((zsc + h * idx - c[0]) ** 2).sum(-1)
+        r = np.sqrt(np.maximum(r2, (h / 4) ** 2))
+        u, _ = _softcore_lj(r, eps, sig, al_r[a])
+        exp = np.where(r2 <= cut * cut, u - uc - duc * (r - cut), 0.0)
+        scale = np.abs(exp).max()
+        assert np.abs(g - exp).max() <= 1e-11 * scale, a
+        # the shifts: energy and force vanish at the cutoff
+        p0 = par[a]
+        A = cut ** 6 + p0[2]
+        f_c = (p0[0] / A - p0[1]) / A + p0[3] + p0[4] * cut
+        assert abs(f_c) <= 1e-12 * scale
+        assert abs(p0[4] + duc) <= 1e-10 * abs(duc)
+
+
+def test_ljavg_library_matches_oracle(oracle):
+    rng = np.random.default_rng(6)
+    args = (rng.uniform(0.1, 0.5, 40), rng.uniform(2.5, 4.0, 40), rng.uniform(0, 1, 40),
+            rng.uniform(0.1, 0.5, 30), rng.uniform(2.5, 4.0, 30))
+    for mix in (0, 1):
+        a = oracle.ljavg_params(*args, mix, 9.0)
+        b = pdfft.ljavg_params(*args, mix=mix, cutoff=9.0)
+        for x, y in zip(a, b):
+            assert np.array_equal(x, y), mix
+
+
 def _case(oracle, n, h, T, seed, n_rec=150, n_lig=30):
     cx = synth.make_complex(n_rec, n_lig, seed)
     rt, w = synth.terms_for(cx, T)
