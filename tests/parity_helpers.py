@@ -111,3 +111,40 @@ def check_peaks_terms(oracle, n, h, cx, rt, w, R, flags, n_rot, K=10, rot_seed=4
     print(f"  checked {checked} of {len(rots) * K} ranks exactly (tol={tol:.3e})", flush=True)
     assert checked >= len(rots) * K // 2
     return checked
+
+
+def check_peaks_measured_band(oracle, n, h, cx, rt, w, R, flags, n_rot, K=10, rot_seed=4, batch=5):
+    """Peaks against the oracle with the band measured on the grids themselves.
+
+    For every rotation the GPU's FP32 correlation grid (pd_scan_correlation_grid:
+    the scan's kernels, whose results do not depend on the batch) is compared
+    with the oracle's FP64 grid: E = max |gpu - oracle| over all cells.  The
+    selection is exact on the GPU's own values, so at every rank whose oracle
+    value is more than 2 E from both neighbours the GPU must report the oracle's
+    cell.  Returns (ranks checked, max E / tol) where tol is the norm model."""
+    S = oracle.receptor_spectra(R)
+    rots = synth.rotation_set(n_rot, seed=rot_seed)
+    tol = tolerance(R, w, n)
+    nout_o, off_o, s_o, val_o, grids = oracle.scan(n, h, S, cx.ligand.pos, w, rots, K=K + 1, max_val=1e16,
+                                                   n_threads=4, want_grids=True)
+    with make_scan(n, h, len(w), K, cx, rt, w, batch=batch, flags=flags) as sc:
+        E = np.array([np.abs(sc.correlation_grid(rots[r]).astype(np.float64) - grids[r]).max()
+                      for r in range(n_rot)])
+        peaks, nout = sc.run(rots, rot_index0=100)
+    assert (nout == K).all()
+    checked = 0
+    for r in range(n_rot):
+        v, band = val_o[r], 2 * E[r]
+        assert (peaks["rot"][r] == 100 + r).all() and (peaks["rank"][r] == np.arange(K)).all()
+        for i in range(K):
+            lo_gap = v[i] - v[i - 1] if i > 0 else np.inf
+            if lo_gap > band and v[i + 1] - v[i] > band:
+                assert peaks["offset"][r, i] == off_o[r, i], (r, i, E[r], peaks[r], off_o[r], v)
+                assert (peaks["s"][r, i] == s_o[r, i]).all()
+                checked += 1
+            assert abs(peaks["value"][r, i] - v[i]) <= E[r] + 1e-12 * abs(v[i]), (r, i, E[r])
+    ratio = float(E.max() / tol)
+    print(f"  measured band: max|E| = {E.max():.3e} = {ratio:.4f} of the norm model tol {tol:.3e}; "
+          f"{checked} of {n_rot * K} ranks outside 2E checked exactly", flush=True)
+    assert ratio <= 1.0
+    return checked, ratio

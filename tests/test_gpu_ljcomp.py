@@ -23,7 +23,7 @@ if not gpu_available():
 from proddl2_amd import synth  # noqa: E402
 
 from tests.parity_helpers import (  # noqa: E402
-    check_grids, check_peaks_terms, make_scan, oracle_receptor, tolerance)
+    check_grids, check_peaks_measured_band, make_scan, oracle_receptor, tolerance)
 
 # grid, h, n_rec, n_lig, alpha (number: every atom; pair: per-atom U(lo, hi), as
 # force_field_fft.py writes one value per atom), cutOffFft, flags
@@ -72,9 +72,14 @@ def test_ljcomp_correlation_grid(oracle, n, h, n_rec, n_lig, alpha, cut, flags):
 
 @pytest.mark.parametrize("n,h,n_rec,n_lig,alpha,cut,flags", LJ_CASES)
 def test_ljcomp_scan_peaks_match_oracle(oracle, n, h, n_rec, n_lig, alpha, cut, flags):
+    """Peak integers exact outside twice the measured grid error, and values
+    within it.  The norm model's band (tol) is what the synthetic terms use; at
+    small alpha it is far wider than the error the grids actually show (the
+    receptor's soft cores dominate ||R||), so it would leave few ranks to check."""
     print(f"case ljComp peaks n={n} alpha={alpha} flags={flags}", flush=True)
     cx, rt, w, R = lj_case(oracle, n, h, n_rec, n_lig, alpha, cut, seed=23)
-    check_peaks_terms(oracle, n, h, cx, rt, w, R, flags, n_rot=12, K=10, rot_seed=8)
+    checked, ratio = check_peaks_measured_band(oracle, n, h, cx, rt, w, R, flags, n_rot=12, K=10, rot_seed=8)
+    assert checked >= 12 * 10 // 2
 
 
 def test_ljcomp_two_copies_double_the_energies(oracle):
