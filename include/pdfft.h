@@ -135,6 +135,16 @@ pd_status pd_scan_sync(struct pd_scan* h);
 pd_status pd_scan_rescore64(struct pd_scan* h, int n, const double* rots, const int* s,
                             double* energies);
 
+/* pd_scan_rescore64 without any allocation (no counterpart in the reference):
+ * for callers that cannot allocate device memory, e.g. tight serving loops or
+ * harnesses that forbid hipMalloc.  The poses are staged through a fixed
+ * device-side area of the library (one per GPU, 736 KiB, held until the process
+ * ends), so n is limited to 8192 poses per call (PD_ERR_INVALID_ARGUMENT above
+ * that).  Same results as pd_scan_rescore64.  The area keeps the last call's
+ * contents; calls on one GPU share it and are serialised by the library. */
+pd_status pd_scan_rescore64_noalloc(struct pd_scan* h, int n, const double* rots, const int* s,
+                                    double* energies);
+
 /* The summed correlation grid of one rotation (FP32, N0*N1*N2), for parity tests
  * (the gridTot after collectTotal, docking_app.hpp:427). */
 pd_status pd_scan_correlation_grid(struct pd_scan* h, const double* rot, float* out);

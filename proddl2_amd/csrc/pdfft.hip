@@ -167,6 +167,28 @@ constexpr int kColThreads = 256;
 constexpr int kMaxLds = 160 * 1024;
 constexpr int kMaxK = 32;
 
+// Staging area of pd_scan_rescore64_noalloc: the poses' rotations, displacements
+// and energies of one call, at most kNoAllocMaxPoses of them.  A file-scope
+// device array, so each GPU holds its own copy from module load to process
+// exit and no call allocates.  In host code `scratch` names a host-side stand-in:
+// kernels and copies use the device address that hipGetSymbolAddress returns
+// for the current device (scratch_ptr).
+constexpr int kNoAllocMaxPoses = 8192;
+constexpr size_t kNoAllocRotOff = 0;                                            // n x 9 f64
+constexpr size_t kNoAllocShiftOff = kNoAllocRotOff + 72 * (size_t)kNoAllocMaxPoses;  // n x 3 i32
+constexpr size_t kNoAllocEnergyOff = kNoAllocShiftOff + 12 * (size_t)kNoAllocMaxPoses;  // n x f64
+constexpr size_t kNoAllocBytes = kNoAllocEnergyOff + 8 * (size_t)kNoAllocMaxPoses;
+static_assert(kNoAllocShiftOff % 256 == 0 && kNoAllocEnergyOff % 256 == 0, "regions stay 256-byte aligned");
+
+// Device address of `scratch` per device, set once per device while a handle is
+// created there (pd_scan_create), and the lock that serialises the calls that
+// use it on that device.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_setup;
+void* g_scratch_ptr[kMaxDevices] = {};
+std::string g_scratch_err[kMaxDevices];
+std::mutex g_scratch_use[kMaxDevices];
+
 int field_nparams(int kind) {
   switch (kind) {
     case PD_FIELD_LJ_REP: return 2;
@@ -237,6 +259,9 @@ bool plan_fits(const HostPlan& P, int lines, int threads) {
 }
 
 }  // namespace
+
+// The no-alloc staging area (see kNoAllocBytes).
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
 
 struct pd_graph;
 
@@ -807,6 +832,20 @@ pd_status pd_scan_create(const pd_scan_desc* desc, pd_scan** out) {
   PD_HIP(hipGetDeviceCount(&ndev));
   if (d.device < 0 || d.device >= ndev) return fail(PD_ERR_DEVICE, "no such HIP device");
   PD_HIP(hipSetDevice(d.device));
+  if (d.device < kMaxDevices) {
+    // once per device, with that device current: the device address of the
+    // no-alloc staging array (a failure only disables pd_scan_rescore64_noalloc,
+    // which then reports it)
+    std::lock_guard<std::mutex> lk(g_scratch_setup);
+    if (!g_scratch_ptr[d.device] && g_scratch_err[d.device].empty()) {
+      void* p = nullptr;
+      const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+      if (e == hipSuccess && p)
+        g_scratch_ptr[d.device] = p;
+      else
+        g_scratch_err[d.device] = std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e);
+    }
+  }
 
   pd_scan* h = new pd_scan();
   h->d = d;
@@ -1176,6 +1215,33 @@ pd_status pd_scan_rescore64(pd_scan* h, int n, const double* rots, const int* s,
   hipFree(dr);
   hipFree(ds);
   hipFree(de);
+  return PD_OK;
+}
+
+pd_status pd_scan_rescore64_noalloc(pd_scan* h, int n, const double* rots, const int* s, double* energies) {
+  if (!h || n < 0 || (n > 0 && (!rots || !s || !energies))) return fail(PD_ERR_INVALID_ARGUMENT, "null argument");
+  if (n > kNoAllocMaxPoses)
+    return fail(PD_ERR_INVALID_ARGUMENT, "pd_scan_rescore64_noalloc takes at most " +
+                                             std::to_string(kNoAllocMaxPoses) + " poses per call (got " +
+                                             std::to_string(n) + "); use pd_scan_rescore64");
+  if (!h->receptor_ready || !h->ligand_ready) return fail(PD_ERR_STATE, "receptor or ligand not set");
+  if (n == 0) return PD_OK;
+  const int dev = h->d.device;
+  if (dev >= kMaxDevices) return fail(PD_ERR_UNSUPPORTED, "device ordinal beyond the staging table");
+  if (!g_scratch_ptr[dev]) return fail(PD_ERR_DEVICE, "no staging area on this device: " + g_scratch_err[dev]);
+  PD_HIP(hipSetDevice(dev));
+  unsigned char* base = static_cast<unsigned char*>(g_scratch_ptr[dev]);
+  double* dr = reinterpret_cast<double*>(base + kNoAllocRotOff);
+  int* ds = reinterpret_cast<int*>(base + kNoAllocShiftOff);
+  double* de = reinterpret_cast<double*>(base + kNoAllocEnergyOff);
+  // one call at a time per device: the staging area is shared by every handle there
+  std::lock_guard<std::mutex> lk(g_scratch_use[dev]);
+  PD_HIP(hipMemcpyAsync(dr, rots, sizeof(double) * 9 * (size_t)n, hipMemcpyHostToDevice, h->stream));
+  PD_HIP(hipMemcpyAsync(ds, s, sizeof(int) * 3 * (size_t)n, hipMemcpyHostToDevice, h->stream));
+  hipLaunchKernelGGL(k_rescore, dim3(n), dim3(256), 0, h->stream, h->g, dr, ds, h->q, h->n_lig, h->w64, h->R64, de);
+  PD_HIP(hipGetLastError());
+  PD_HIP(hipMemcpyAsync(energies, de, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost, h->stream));
+  PD_HIP(hipStreamSynchronize(h->stream));
   return PD_OK;
 }
 

@@ -36,6 +36,7 @@ EXPORTS = (
     "pd_scan_kernel_stats", "pd_scan_stream", "pd_scan_destroy", "pd_last_error",
     "pd_find_best_size", "pd_ljcomp_params", "pd_scan_set_batch", "pd_graph_create", "pd_graph_launch",
     "pd_graph_sync", "pd_graph_stream", "pd_graph_destroy", "pd_ljavg_params",
+    "pd_scan_rescore64_noalloc",
 )
 
 KERNEL_NAMES = ("k_bin", "k_lig_plane_fwd", "k_col", "k_plane_inv_topk", "k_merge")
@@ -117,6 +118,7 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     lib.pd_scan_run_device.argtypes = [P, i, P, i, P, P, i]
     lib.pd_scan_sync.argtypes = [P]
     lib.pd_scan_rescore64.argtypes = [P, i, dp, ip, dp]
+    lib.pd_scan_rescore64_noalloc.argtypes = [P, i, dp, ip, dp]
     lib.pd_scan_correlation_grid.argtypes = [P, dp, C.POINTER(C.c_float)]
     lib.pd_scan_set_timing.argtypes = [P, i]
     lib.pd_scan_kernel_stats.argtypes = [P, i, dp, C.POINTER(C.c_longlong)]
