@@ -447,7 +447,7 @@ pd_status resolve_timing(pd_scan* h) {
 }
 
 int plane_lds(const pd_scan* h) { return h->g.n1 * h->g.ls * (int)sizeof(float2); }
-int split_row_lds(const pd_scan* h) { return kSplitRows * h->g.ls * (int)sizeof(float2); }
+int split_row_lds(const pd_scan* h) { return h->g.zrows * h->g.ls * (int)sizeof(float2); }
 int split_col_lds(const pd_scan* h) { return h->g.n1 * kSplitColStride * (int)sizeof(float2); }
 int col_lds(const pd_scan* h) { return h->g.n0 * h->C * (int)sizeof(float2); }
 
@@ -465,7 +465,7 @@ pd_status receptor_spectra(pd_scan* h) {
   const int ntile = (g.M + h->C - 1) / h->C;
   for (int t = 0; t < g.T; ++t) {
     if (h->split) {
-      hipLaunchKernelGGL(k_rec_rows, dim3(g.n0, (g.n1 + kSplitRows - 1) / kSplitRows), dim3(kSplitThreads),
+      hipLaunchKernelGGL(k_rec_rows, dim3(g.n0, (g.n1 + g.zrows - 1) / g.zrows), dim3(kSplitThreads),
                          split_row_lds(h), h->stream, g, h->pz, h->tw2, r32 + (size_t)t * h->N, tmp);
       if (h->fycol)
         hipLaunchKernelGGL(h->fycol->fwd, dim3(g.n0, (g.h2 + h->fycol->cols - 1) / h->fycol->cols), dim3(256),
@@ -518,7 +518,7 @@ pd_status launch_fft_path(pd_scan* h, const pd_scan::Lane& ln, int nb, const dou
       L(kLigSplit, dim3((g.p[0] + 15) / 16 * 16, nb, g.T), dim3(1024), kInvSplitLds, g, h->tw_y, h->tw_z, h->tw2,
                          ln.cells, h->n_lig, h->w32, ln.L1);
     } else if (h->split || h->lig_rows) {
-      L(k_lig_rows, dim3(g.p[0], nb, (g.p[1] + kSplitRows - 1) / kSplitRows),
+      L(k_lig_rows, dim3(g.p[0], nb, (g.p[1] + g.zrows - 1) / g.zrows),
                          dim3(kSplitThreads), split_row_lds(h), g, h->pz, h->tw2, ln.cells, h->n_lig,
                          h->w32, ln.L1);
       if (h->fycol)
@@ -820,7 +820,12 @@ pd_status pd_scan_create(const pd_scan_desc* desc, pd_scan** out) {
   const pd_scan_desc& d = *desc;
   for (int i = 0; i < 3; ++i)
     if (d.n[i] < 2 || d.n[i] > 4096) return fail(PD_ERR_INVALID_ARGUMENT, "grid edge out of range");
-  if (d.n[2] % 2 != 0) return fail(PD_ERR_UNSUPPORTED, "odd last grid edge is not supported");
+  {
+    std::vector<int> r;
+    const int zlen = d.n[2] % 2 ? d.n[2] : d.n[2] / 2;  // length of the z transform
+    if (!radices_for(d.n[0], r) || !radices_for(d.n[1], r) || !radices_for(zlen, r))
+      return fail(PD_ERR_UNSUPPORTED, "grid edges must factor over {2,3,5,7,11,13}");
+  }
   if (!(d.grid_step > 0)) return fail(PD_ERR_INVALID_ARGUMENT, "grid_step must be > 0");
   if (d.n_terms < 1 || d.n_terms > 16) return fail(PD_ERR_INVALID_ARGUMENT, "n_terms must be 1..16");
   if (d.max_n_trans < 1 || d.max_n_trans > kMaxK)
@@ -856,8 +861,9 @@ pd_status pd_scan_create(const pd_scan_desc* desc, pd_scan** out) {
   g.n1 = d.n[1];
   g.n2 = d.n[2];
   g.h2 = g.n2 / 2 + 1;
-  g.m2 = g.n2 / 2;
-  g.ls = g.h2 | 1;
+  g.odd2 = g.n2 % 2;
+  g.m2 = g.odd2 ? g.n2 : g.n2 / 2;  // the z transform: full complex length for an odd n2
+  g.ls = g.odd2 ? g.n2 : (g.h2 | 1);
   g.M = g.n1 * g.h2;
   g.T = d.n_terms;
   g.h = d.grid_step;
@@ -895,11 +901,15 @@ pd_status pd_scan_create(const pd_scan_desc* desc, pd_scan** out) {
   if ((st = make_twiddles(h, g.n2, &h->tw2)) != PD_OK) return bad(st, pd_detail::last_error());
   if (!make_plan(g.n0, h->hx) || !make_plan(g.n1, h->hy) || !make_plan(g.m2, h->hz))
     return bad(PD_ERR_UNSUPPORTED, "grid edges must factor over {2,3,5,7,11,13}");
+  // rows per block of the split row kernels: 32, halved while the z plan does
+  // not fit the threads' registers (long odd z transforms such as 243 = 3^5)
+  g.zrows = kSplitRows;
+  while (g.zrows > 1 && !plan_fits(h->hz, g.zrows, kSplitThreads)) g.zrows /= 2;
   h->split = (d.flags & PD_SCAN_SPLIT_PLANES) != 0 || plane_lds(h) > kMaxLds ||
              !plan_fits(h->hy, g.h2, kPlaneThreads) || !plan_fits(h->hz, g.n1, kPlaneThreads);
   if (!plan_fits(h->hx, h->C, kColThreads) || g.n0 * h->C > PD_COL_PER * kColThreads)
     return bad(PD_ERR_UNSUPPORTED, "grid edge too large for the column register plan");
-  if (h->split && (!plan_fits(h->hz, kSplitRows, kSplitThreads) || !plan_fits(h->hy, kSplitCols, kSplitThreads) ||
+  if (h->split && (!plan_fits(h->hz, g.zrows, kSplitThreads) || !plan_fits(h->hy, kSplitCols, kSplitThreads) ||
                    split_row_lds(h) > kMaxLds || split_col_lds(h) > kMaxLds))
     return bad(PD_ERR_UNSUPPORTED, "grid edge too large for the split plane kernels");
   {
@@ -919,7 +929,7 @@ pd_status pd_scan_create(const pd_scan_desc* desc, pd_scan** out) {
   // CU: 144^3, 160^3) runs the ligand pass as the split row and column
   // kernels, which measured twice as fast there (160^3: 32.9 -> 15.4 us per
   // rotation); its inverse keeps the whole-plane kernel.
-  h->lig_rows = !h->split && 2 * plane_lds(h) > kMaxLds && plan_fits(h->hz, kSplitRows, kSplitThreads) &&
+  h->lig_rows = !h->split && 2 * plane_lds(h) > kMaxLds && plan_fits(h->hz, g.zrows, kSplitThreads) &&
                 plan_fits(h->hy, kSplitCols, kSplitThreads) && split_row_lds(h) <= kMaxLds &&
                 split_col_lds(h) <= kMaxLds;
   if (h->lig_rows) {

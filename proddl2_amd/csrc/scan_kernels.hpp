@@ -29,8 +29,14 @@
 struct ScanGeom {
   int n0, n1, n2;
   int h2;       // n2/2 + 1
-  int m2;       // n2/2 (length of the packed half-length complex FFT)
-  int ls;       // LDS row stride (complex) in the plane kernels, odd, >= h2
+  // Length of the complex FFT along z: n2/2 for an even n2 (two reals packed
+  // per complex value, then the r2c post-step), n2 for an odd n2 (the real row
+  // as complex values with imaginary parts 0; the first h2 outputs are the r2c
+  // result).
+  int m2;
+  int odd2;     // n2 is odd
+  int ls;       // LDS row stride (complex) in the plane kernels, odd, >= h2 (>= n2 when odd2)
+  int zrows;    // rows per block of the split path's row kernels (<= 32, so that the z plan fits)
   int M;        // n1 * h2: size of one (k1,k2) frequency plane
   int T;        // terms
   int lo[3];    // ligand box: cells [lo, lo+p) along each axis
@@ -55,6 +61,37 @@ struct ScanGeom {
   double zero_h[3];  // centred-geometry zero_ - h per axis
   double zsc[3];     // centred-geometry cell-0 centre per axis
 };
+
+// Float offset of real element i2 within an LDS row of the z transform:
+// packed two per complex value for an even n2, the real part of slot i2 for an
+// odd n2.
+__device__ __forceinline__ int zreal(const ScanGeom& g, int i2) { return g.odd2 ? 2 * i2 : i2; }
+
+// After the forward z FFT of `rows` rows: the r2c post-step for an even n2;
+// nothing for an odd n2 (the full complex FFT already holds the half spectrum).
+__device__ __forceinline__ void r2c_finish(const ScanGeom& g, float2* buf, int rows, const float2* tw2) {
+  if (!g.odd2) r2c_post(buf, rows, g.ls, g.m2, tw2);
+}
+
+// Before the backward z FFT of `rows` rows holding X[0..n2/2]: the c2r
+// pre-step for an even n2; for an odd n2 the Hermitian completion
+// X[n2-k] = conj X[k] (k = 1..n2/2) with Im X[0] ignored, as FFTW's c2r does.
+__device__ __forceinline__ void c2r_start(const ScanGeom& g, float2* buf, int rows, const float2* tw2) {
+  if (!g.odd2) {
+    c2r_pre(buf, rows, g.ls, g.m2, tw2);
+    return;
+  }
+  const int hz = g.n2 / 2;
+  for (int task = threadIdx.x; task < rows * (hz + 1); task += blockDim.x) {
+    const int row = task % rows, k = task / rows;
+    float2* z = buf + row * g.ls;
+    if (k == 0)
+      z[0].y = 0.f;
+    else
+      z[g.n2 - k] = cconj(z[k]);  // writes slots hz+1..n2-1, reads 1..hz
+  }
+  __syncthreads();
+}
 
 // Offset (complex) of plane q, column m within one (rotation, term) of L1.
 __host__ __device__ __forceinline__ size_t l1_off(const ScanGeom& g, int q, int m) {
@@ -203,12 +240,12 @@ __global__ __launch_bounds__(512) void k_lig_plane_fwd(ScanGeom g, FftPlan pz, F
       if (off >= 0 && off / plane == i0) {
         const int r = off - i0 * plane;
         const int i1 = r / g.n2, i2 = r - i1 * g.n2;
-        atomicAdd(&fb[(size_t)i1 * 2 * g.ls + i2], wt[j]);
+        atomicAdd(&fb[(size_t)i1 * 2 * g.ls + zreal(g, i2)], wt[j]);
       }
     }
     __syncthreads();
     fft_lds<-1>(buf + g.lo[1] * g.ls, pz, g.p[1], g.ls, 1);
-    r2c_post(buf + g.lo[1] * g.ls, g.p[1], g.ls, g.m2, tw2);
+    r2c_finish(g, buf + g.lo[1] * g.ls, g.p[1], tw2);
     fft_lds<-1>(buf, py, g.h2, 1, g.ls);
     float2* dst = L1 + ((size_t)b * g.T + t) * g.pp * (size_t)g.M;
     const int q = g.lo[0] - g.pl + p;
@@ -322,7 +359,7 @@ __global__ __launch_bounds__(512) void k_plane_inv_topk(ScanGeom g, FftPlan pz, 
   }
   __syncthreads();
   fft_lds<+1>(buf, py, g.h2, 1, g.ls);
-  c2r_pre(buf, g.n1, g.ls, g.m2, tw2);
+  c2r_start(g, buf, g.n1, tw2);
   fft_lds<+1>(buf, pz, g.n1, g.ls, 1);
   float* fb = reinterpret_cast<float*>(buf);
   const int npl = g.n1 * g.n2;
@@ -331,10 +368,10 @@ __global__ __launch_bounds__(512) void k_plane_inv_topk(ScanGeom g, FftPlan pz, 
   int be = INT_MAX;
   for (int e = threadIdx.x; e < npl; e += blockDim.x) {
     const int i1 = e / g.n2, i2 = e - i1 * g.n2;
-    float v = fb[i1 * rs + i2] * scale;
+    float v = fb[i1 * rs + zreal(g, i2)] * scale;
     if (grid_out) grid_out[((size_t)b * g.n0 + i0) * npl + e] = v;
     if (v != v) v = __int_as_float(0x7f800000);
-    fb[i1 * rs + i2] = v;
+    fb[i1 * rs + zreal(g, i2)] = v;
     if (pd_better(v, e, bv, be)) { bv = v; be = e; }
   }
   __syncthreads();
@@ -350,12 +387,12 @@ __global__ __launch_bounds__(512) void k_plane_inv_topk(ScanGeom g, FftPlan pz, 
     }
     if (o != INT_MAX && o == be) {
       const int i1 = o / g.n2, i2 = o - i1 * g.n2;
-      fb[i1 * rs + i2] = __int_as_float(0x7f800000);
+      fb[i1 * rs + zreal(g, i2)] = __int_as_float(0x7f800000);
       bv = __int_as_float(0x7f800000);
       be = INT_MAX;
       for (int e = threadIdx.x; e < npl; e += blockDim.x) {
         const int j1 = e / g.n2, j2 = e - j1 * g.n2;
-        const float x = fb[j1 * rs + j2];
+        const float x = fb[j1 * rs + zreal(g, j2)];
         if (x != __int_as_float(0x7f800000) && pd_better(x, e, bv, be)) { bv = x; be = e; }
       }
     }
@@ -506,11 +543,11 @@ __global__ __launch_bounds__(512) void k_rec_plane_fwd(ScanGeom g, FftPlan pz, F
   __syncthreads();
   for (int e = threadIdx.x; e < npl; e += blockDim.x) {
     const int i1 = e / g.n2, i2 = e - i1 * g.n2;
-    fb[(size_t)i1 * 2 * g.ls + i2] = R[(size_t)i0 * npl + e];
+    fb[(size_t)i1 * 2 * g.ls + zreal(g, i2)] = R[(size_t)i0 * npl + e];
   }
   __syncthreads();
   fft_lds<-1>(buf, pz, g.n1, g.ls, 1);
-  r2c_post(buf, g.n1, g.ls, g.m2, tw2);
+  r2c_finish(g, buf, g.n1, tw2);
   fft_lds<-1>(buf, py, g.h2, 1, g.ls);
   float2* dst = out + (size_t)i0 * g.M;
   for (int e = threadIdx.x; e < g.M; e += blockDim.x) {

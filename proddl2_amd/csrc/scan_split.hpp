@@ -25,12 +25,12 @@
 #include "scan_fast.hpp"
 #include "scan_kernels.hpp"
 
-constexpr int kSplitRows = 32;     // rows per block in the row kernels
+constexpr int kSplitRows = 32;     // rows per block in the row kernels (at most; ScanGeom::zrows)
 constexpr int kSplitCols = 16;     // columns per block in the column kernels
 constexpr int kSplitColStride = kSplitCols + 1;
 constexpr int kSplitThreads = 256;
 
-// grid (p0, nrot, ceil(p1 / kSplitRows)), LDS kSplitRows * ls complex.
+// grid (p0, nrot, ceil(p1 / zrows)), LDS zrows * ls complex.
 __global__ __launch_bounds__(kSplitThreads) void k_lig_rows(ScanGeom g, FftPlan pz,
                                                             const float2* __restrict__ tw2,
                                                             const int* __restrict__ cells, int n_atoms,
@@ -38,13 +38,13 @@ __global__ __launch_bounds__(kSplitThreads) void k_lig_rows(ScanGeom g, FftPlan 
                                                             float2* __restrict__ L1) {
   extern __shared__ float2 buf[];
   const int p = blockIdx.x, b = blockIdx.y;
-  const int r0 = g.lo[1] + blockIdx.z * kSplitRows;
-  const int nr = min(kSplitRows, g.lo[1] + g.p[1] - r0);
+  const int r0 = g.lo[1] + blockIdx.z * g.zrows;
+  const int nr = min(g.zrows, g.lo[1] + g.p[1] - r0);
   const int i0 = g.lo[0] + p;
   const int plane = g.n1 * g.n2;
   const int* cb = cells + (size_t)b * n_atoms;
   float* fb = reinterpret_cast<float*>(buf);
-  const int tot = kSplitRows * g.ls;
+  const int tot = g.zrows * g.ls;
   for (int t = 0; t < g.T; ++t) {
     for (int e = threadIdx.x; e < tot; e += blockDim.x) buf[e] = make_float2(0.f, 0.f);
     __syncthreads();
@@ -54,12 +54,12 @@ __global__ __launch_bounds__(kSplitThreads) void k_lig_rows(ScanGeom g, FftPlan 
       if (off >= 0 && off / plane == i0) {
         const int r = off - i0 * plane;
         const int i1 = r / g.n2, i2 = r - i1 * g.n2;
-        if (i1 >= r0 && i1 < r0 + nr) atomicAdd(&fb[(size_t)(i1 - r0) * 2 * g.ls + i2], wt[j]);
+        if (i1 >= r0 && i1 < r0 + nr) atomicAdd(&fb[(size_t)(i1 - r0) * 2 * g.ls + zreal(g, i2)], wt[j]);
       }
     }
     __syncthreads();
     fft_lds<-1>(buf, pz, nr, g.ls, 1);
-    r2c_post(buf, nr, g.ls, g.m2, tw2);
+    r2c_finish(g, buf, nr, tw2);
     float2* dst = L1 + (((size_t)b * g.T + t) * g.pp + (g.lo[0] - g.pl) + p) * (size_t)g.M + (size_t)r0 * g.h2;
     for (int e = threadIdx.x; e < nr * g.h2; e += blockDim.x) {
       const int rr = e / g.h2, k2 = e - rr * g.h2;
@@ -176,25 +176,25 @@ __global__ __launch_bounds__(256) void k_cols_fast(ScanGeom g, const float2* __r
   }
 }
 
-// Receptor rows: dense real rows -> r2c(z).  grid (n0, ceil(n1 / kSplitRows)).
+// Receptor rows: dense real rows -> r2c(z).  grid (n0, ceil(n1 / zrows)).
 __global__ __launch_bounds__(kSplitThreads) void k_rec_rows(ScanGeom g, FftPlan pz, const float2* __restrict__ tw2,
                                                             const float* __restrict__ R,
                                                             float2* __restrict__ out) {
   extern __shared__ float2 buf[];
   const int i0 = blockIdx.x;
-  const int r0 = blockIdx.y * kSplitRows;
-  const int nr = min(kSplitRows, g.n1 - r0);
+  const int r0 = blockIdx.y * g.zrows;
+  const int nr = min(g.zrows, g.n1 - r0);
   float* fb = reinterpret_cast<float*>(buf);
-  for (int e = threadIdx.x; e < kSplitRows * g.ls; e += blockDim.x) buf[e] = make_float2(0.f, 0.f);
+  for (int e = threadIdx.x; e < g.zrows * g.ls; e += blockDim.x) buf[e] = make_float2(0.f, 0.f);
   __syncthreads();
   const float* src = R + ((size_t)i0 * g.n1 + r0) * g.n2;
   for (int e = threadIdx.x; e < nr * g.n2; e += blockDim.x) {
     const int rr = e / g.n2, i2 = e - rr * g.n2;
-    fb[(size_t)rr * 2 * g.ls + i2] = src[e];
+    fb[(size_t)rr * 2 * g.ls + zreal(g, i2)] = src[e];
   }
   __syncthreads();
   fft_lds<-1>(buf, pz, nr, g.ls, 1);
-  r2c_post(buf, nr, g.ls, g.m2, tw2);
+  r2c_finish(g, buf, nr, tw2);
   float2* dst = out + (size_t)i0 * g.M + (size_t)r0 * g.h2;
   for (int e = threadIdx.x; e < nr * g.h2; e += blockDim.x) {
     const int rr = e / g.h2, k2 = e - rr * g.h2;
@@ -202,7 +202,7 @@ __global__ __launch_bounds__(kSplitThreads) void k_rec_rows(ScanGeom g, FftPlan 
   }
 }
 
-// Inverse rows + per-plane top-K.  grid (n0, nrot), LDS kSplitRows * ls complex.
+// Inverse rows + per-plane top-K.  grid (n0, nrot), LDS zrows * ls complex.
 // The plane's K best (value, offset) pairs are kept sorted in LDS while the
 // chunks of rows stream through.
 __global__ __launch_bounds__(kSplitThreads) void k_inv_rows_topk(ScanGeom g, FftPlan pz,
@@ -226,25 +226,25 @@ __global__ __launch_bounds__(kSplitThreads) void k_inv_rows_topk(ScanGeom g, Fft
     lv[threadIdx.x] = INF;
     lo[threadIdx.x] = INT_MAX;
   }
-  for (int r0 = 0; r0 < g.n1; r0 += kSplitRows) {
-    const int nr = min(kSplitRows, g.n1 - r0);
+  for (int r0 = 0; r0 < g.n1; r0 += g.zrows) {
+    const int nr = min(g.zrows, g.n1 - r0);
     __syncthreads();
     for (int e = threadIdx.x; e < nr * g.h2; e += blockDim.x) {
       const int rr = e / g.h2, k2 = e - rr * g.h2;
       buf[rr * g.ls + k2] = src[(size_t)r0 * g.h2 + e];
     }
     __syncthreads();
-    c2r_pre(buf, nr, g.ls, g.m2, tw2);
+    c2r_start(g, buf, nr, tw2);
     fft_lds<+1>(buf, pz, nr, g.ls, 1);
     float bv = INF;
     int be = INT_MAX;
     for (int e = threadIdx.x; e < nr * g.n2; e += blockDim.x) {
       const int rr = e / g.n2, i2 = e - rr * g.n2;
       const int o = r0 * g.n2 + e;
-      float v = fb[rr * rs + i2] * scale;
+      float v = fb[rr * rs + zreal(g, i2)] * scale;
       if (grid_out) grid_out[((size_t)b * g.n0 + i0) * npl + o] = v;
       if (v != v) v = INF;
-      fb[rr * rs + i2] = v;
+      fb[rr * rs + zreal(g, i2)] = v;
       if (pd_better(v, o, bv, be)) { bv = v; be = o; }
     }
     __syncthreads();
@@ -266,12 +266,12 @@ __global__ __launch_bounds__(kSplitThreads) void k_inv_rows_topk(ScanGeom g, Fft
       }
       if (o == be) {
         const int rr = o / g.n2 - r0, i2 = o % g.n2;
-        fb[rr * rs + i2] = INF;
+        fb[rr * rs + zreal(g, i2)] = INF;
         bv = INF;
         be = INT_MAX;
         for (int e = threadIdx.x; e < nr * g.n2; e += blockDim.x) {
           const int r2 = e / g.n2, j2 = e - r2 * g.n2;
-          const float x = fb[r2 * rs + j2];
+          const float x = fb[r2 * rs + zreal(g, j2)];
           const int oe = r0 * g.n2 + e;
           if (x != INF && pd_better(x, oe, bv, be)) { bv = x; be = oe; }
         }

@@ -85,12 +85,15 @@ def test_errors_map_to_status_codes():
     # Argument validation happens on the host before any device call.
     lib = pdfft.load()
     d = pdfft.ScanDesc()
-    d.n[:] = [32, 32, 31]
+    d.n[:] = [32, 32, 34]
     d.grid_step, d.n_terms, d.max_n_trans, d.max_n_trans_inp = 1.0, 2, 4, 4
     import ctypes as C
     h = C.c_void_p()
-    assert lib.pd_scan_create(C.byref(d), C.byref(h)) == 5  # odd last edge: unsupported
-    d.n[2] = 32
+    assert lib.pd_scan_create(C.byref(d), C.byref(h)) == 5  # 34 = 2 x 17: not a findBestSize edge
+    assert b"factor" in lib.pd_last_error()
+    d.n[:] = [51, 32, 32]  # 3 x 17
+    assert lib.pd_scan_create(C.byref(d), C.byref(h)) == 5
+    d.n[0] = 32
     d.max_n_trans = 40
     assert lib.pd_scan_create(C.byref(d), C.byref(h)) == 5
     d.max_n_trans = 4

@@ -43,6 +43,14 @@ CASES = [
     ((160, 160, 160), 1.2, 2000, 500, 2),
     # the same choice behind the runtime-plan kernels (150 x 76 plane: 92 KB of LDS)
     ((40, 150, 150), 1.5, 300, 80, 2),
+    # odd last edges, as findBestSize returns them (the z transform is a full-length
+    # complex FFT; the column pass runs the N0-specialised kernel at 64)
+    ((45, 50, 75), 1.5, 300, 80, 2),
+    ((77, 78, 91), 1.2, 1500, 300, 4),
+    ((75, 75, 75), 1.2, 1500, 300, 2),
+    ((64, 60, 81), 1.5, 700, 200, 4),
+    # 135 x 135 plane (146 KB of LDS): ligand pass as row and column kernels
+    ((135, 135, 135), 1.2, 2000, 500, 2),
 ]
 
 
@@ -66,6 +74,9 @@ CASES_BOTH = [c + (0,) for c in CASES] + [
     ((96, 96, 96), 1.2, 1500, 300, 4, SPLIT),
     # z half-length 26 has no register-FFT split: the runtime-plan row kernel
     ((56, 44, 52), 1.5, 300, 80, 2, SPLIT),
+    # odd last edges through the row and column kernels; 243 = 3^5 takes 16 rows per block
+    ((40, 36, 45), 1.5, 300, 80, 4, SPLIT),
+    ((32, 40, 243), 1.0, 300, 80, 2, SPLIT),
     # N1 = N2 = 256 (a 256 x 129 plane exceeds LDS): the ligand plane pass and the
     # inverse (y pass, rows, selection) each in one kernel, and the same case
     # through the separate row and column kernels
@@ -243,3 +254,16 @@ def test_call_order_is_state_error():
         with pytest.raises(pdfft.PdError) as ei:
             sc.run(synth.rotation_set(1, seed=1))
         assert ei.value.status == 6
+
+
+def test_every_find_best_size_edge_is_accepted():
+    """Every findBestSize edge up to 256 (Math/fft.hpp:141-168: {2,3,5,7,11,13}-smooth
+    with e11 + e13 <= 1, odd ones included) is accepted on each axis."""
+    edges = sorted({pdfft.find_best_size(k) for k in range(8, 257)} - {257})
+    edges = [e for e in edges if e <= 256]
+    assert 75 in edges and 91 in edges and 135 in edges and 243 in edges
+    for e in edges:
+        for n in ((e, 32, 32), (32, e, 32), (32, 32, e)):
+            with pdfft.Scan(n, 1.5, 2, max_n_trans=4, batch=4):
+                pass
+    print(f"  {len(edges)} edges accepted on every axis", flush=True)
