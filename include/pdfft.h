@@ -162,6 +162,40 @@ void* pd_scan_stream(struct pd_scan* h);
 pd_status pd_scan_destroy(struct pd_scan* h);
 const char* pd_last_error(void);
 
+/* Post-filters of each rotation's translations (Worker::init,
+ * docking_app.hpp:363-398; applied in findTranslationalMinima, :417-447):
+ * the scan keeps the max_n_trans_inp best translations (CorrelationProcessor),
+ * then TranSymm drops those whose pose is not C_n-symmetric
+ * (docking_fft.hpp:146-230), then TranClust keeps cluster leaders
+ * (docking_fft.hpp:30-140), and the first max_n_trans of what is left are
+ * reported.  Rigid transformations are 12 doubles: rotation (3x3 row-major),
+ * then translation.  With TranClust the reported leaders come in the
+ * reference's order: from the highest energy down (ClusterMatrix,
+ * Geom/cluster.hpp:291-309). */
+typedef struct pd_postfilter {
+  int n_multimer;           /* option nMultimer: TranSymm when > 1                               */
+  double max_rmsd_symm;     /* option maxRmsdSymm (reference default 8.0)                        */
+  double ligand_radius;     /* MolStruct::getSizeLigand() / 2: radius of the symmetry test points */
+  double frame_rec_inv[12]; /* tR^-1 of ToOriginalFrameTransformer (docking_molstruct.hpp:31-51)  */
+  double frame_lig[12];     /* tL of ToOriginalFrameTransformer                                  */
+  int cluster;              /* option doClusterTranslations: TranClust when != 0                  */
+  double cluster_radius;    /* option clusterRadiusTrans (reference default 5.0)                  */
+} pd_postfilter;
+
+/* Sets (pf non-null with n_multimer > 1 or cluster != 0) or clears the
+ * post-filters of pd_scan_run / pd_scan_run_device / graphs.  They need
+ * max_n_trans_inp <= 4096; each batch then also keeps its real correlation
+ * grids (batch x N0 N1 N2 floats per batch lane). */
+pd_status pd_scan_set_postfilter(struct pd_scan* h, const pd_postfilter* pf);
+
+/* The max_n_trans_inp best translations of each rotation before any
+ * post-filter, ascending (CorrelationProcessor::selectFromFFT +
+ * getTranValues, docking_fft.hpp:316-345, 409-414): peaks is
+ * n_rot x max_n_trans_inp (host), n_out[i] the number found (fewer when fewer
+ * cells lie below max_val_corr).  max_n_trans_inp <= 4096. */
+pd_status pd_scan_candidates(struct pd_scan* h, int n_rot, const double* rots, int rot_index0,
+                             pd_peak* peaks, int* n_out);
+
 /* Rotations per kernel launch (pd_scan_desc.batch), changed after creation;
  * reallocates the per-batch buffers.  Results do not depend on it.
  * PD_ERR_STATE while a graph built from the handle exists (as for

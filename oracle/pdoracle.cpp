@@ -672,6 +672,166 @@ int pdo_scan(const int* n, double h, int n_terms, const double* rec_spectra, int
   return 0;
 }
 
+// ---------------------------------------------------------------------------
+// Post-filters (TranSymm, TranClust) on one rotation's translations.
+// ---------------------------------------------------------------------------
+
+namespace {
+
+// RotationTranslation: x -> m x + v (Geom/transformation.hpp:244-360).
+struct Rigid {
+  double m[9];
+  double v[3];
+};
+
+Rigid rigid_from(const double* t12) {
+  Rigid r;
+  for (int i = 0; i < 9; ++i) r.m[i] = t12[i];
+  for (int i = 0; i < 3; ++i) r.v[i] = t12[9 + i];
+  return r;
+}
+
+// blitz_ext::product(tensor, vector) (bztinyvec.hpp:52-65)
+void product_mv(const double* m, const double* x, double* y) {
+  for (int i = 0; i < 3; ++i) {
+    double acc = 0;
+    for (int j = 0; j < 3; ++j) acc += m[3 * i + j] * x[j];
+    y[i] = acc;
+  }
+}
+
+// RotationTranslation::operator*(RotationTranslation) (transformation.hpp:323-328):
+// m = a.m b.m, v = a.m b.v + a.v
+Rigid compose(const Rigid& a, const Rigid& b) {
+  Rigid r;
+  pdo_matmul3(a.m, b.m, r.m);
+  double p[3];
+  product_mv(a.m, b.v, p);
+  for (int i = 0; i < 3; ++i) r.v[i] = p[i] + a.v[i];
+  return r;
+}
+
+// SymmetryCheckerCn::operator() (Geom/symmetry.hpp:35-51)
+double symm_rmsd(const Rigid& t, int n_symm, double radius) {
+  Rigid tn = t;
+  for (int i = 1; i < n_symm; ++i) tn = compose(t, tn);
+  const double u[4][3] = {{radius, 0, 0}, {0, radius, 0}, {0, 0, radius}, {0, 0, 0}};
+  double sum = 0;
+  for (int k = 0; k < 4; ++k) {
+    double y[3];
+    product_mv(tn.m, u[k], y);  // RotationTranslation::operator()(Point), :330-332
+    double d2 = 0;
+    for (int i = 0; i < 3; ++i) {
+      const double d = (y[i] + tn.v[i]) - u[k][i];
+      d2 += d * d;
+    }
+    sum += d2;
+  }
+  return std::sqrt(sum / 4);
+}
+
+}  // namespace
+
+int pdo_tran_symm(int n, double* tran, double* values, int* tag, const double* rot, const double* frame_a,
+                  const double* frame_b, int n_symm, double mol_radius, double max_rmsd, double* rmsd) {
+  const Rigid A = rigid_from(frame_a), B = rigid_from(frame_b);
+  Rigid R;
+  for (int i = 0; i < 9; ++i) R.m[i] = rot[i];
+  int out = 0;
+  for (int i = 0; i < n; ++i) {
+    // Translation(tran) * Rotation -> RotationTranslation(rot, tran) (transformation.hpp:84-86),
+    // then m_tR * t * m_tL, left to right (docking_molstruct.hpp:41-43)
+    for (int d = 0; d < 3; ++d) R.v[d] = tran[3 * i + d];
+    const Rigid t = compose(compose(A, R), B);
+    const double r = symm_rmsd(t, n_symm, mol_radius);
+    if (rmsd) rmsd[i] = r;
+    if (r < max_rmsd) {
+      for (int d = 0; d < 3; ++d) tran[3 * out + d] = tran[3 * i + d];
+      values[out] = values[i];
+      if (tag) tag[out] = tag[i];
+      ++out;
+    }
+  }
+  return out;
+}
+
+int pdo_tran_clust(int n, double* tran, double* values, int* tag, double cluster_radius, int max_out,
+                   int cap) {
+  if (n <= 0) return 0;
+  // weights = -value (docking_fft.hpp:71); the density is then overwritten by
+  // the weights (cluster.hpp:291-292), so rows are visited in ascending -value
+  const double r2c = cluster_radius * cluster_radius;
+  const double cutoff2 = r2c * 3;                   // cluster.hpp:160-162 (M_col = 3)
+  const double cutoff = std::sqrt(cutoff2);
+  const double rad2 = cutoff * cutoff;              // SphereIter: radius = cutoff (partpoints.hpp:551-562)
+  std::vector<double> dens(n);
+  for (int i = 0; i < n; ++i) dens[i] = -values[i];
+  std::vector<int> ind(n), place(n), leader(n);
+  for (int i = 0; i < n; ++i) ind[i] = leader[i] = i;
+  std::sort(ind.begin(), ind.end(), [&](int a, int b) { return dens[a] < dens[b]; });  // :308-309
+  for (int i = 0; i < n; ++i) place[ind[i]] = i;
+  std::vector<int> leaders;
+  for (int id = 0; id < n; ++id) {
+    const int i = ind[id];
+    int best = place[i], lead = i;
+    // every leader inserted so far within the radius (dist2 <= rad2,
+    // partpoints.hpp:584) is a candidate; the earliest in density order wins
+    for (int k : leaders) {
+      double d2 = 0;
+      for (int d = 0; d < 3; ++d) {
+        const double t = tran[3 * i + d] - tran[3 * k + d];
+        d2 += t * t;
+      }
+      if (d2 <= rad2 && place[k] < best) {
+        best = place[k];
+        lead = k;
+      }
+    }
+    if (lead == i)
+      leaders.push_back(i);
+    else
+      leader[i] = lead;
+  }
+  // getClusters (cluster.hpp:395-414): leaders in density order; then
+  // TranClust::process copies them to the front (docking_fft.hpp:95-112)
+  int n_out = (int)leaders.size();
+  n_out = std::min(n_out, cap);
+  n_out = std::min(n_out, max_out);
+  const std::vector<double> t3(tran, tran + 3 * (size_t)n), v(values, values + n);
+  std::vector<int> tg;
+  if (tag) tg.assign(tag, tag + n);
+  for (int c = 0; c < n_out; ++c) {
+    const int r = leaders[c];
+    for (int d = 0; d < 3; ++d) tran[3 * c + d] = t3[3 * r + d];
+    values[c] = v[r];
+    if (tag) tag[c] = tg[r];
+  }
+  return n_out;
+}
+
+int pdo_select_post(const int* n, double h, const double* grid, int max_inp, double max_val, int K,
+                    const double* rot, const double* frame_a, const double* frame_b, int n_symm,
+                    double mol_radius, double max_rmsd, double cluster_radius, int* offsets, int* s,
+                    double* values) {
+  std::vector<int> off(max_inp), ss(3 * (size_t)max_inp);
+  std::vector<double> tran(3 * (size_t)max_inp), val(max_inp);
+  int m = select(n, h, grid, max_inp, max_val, max_inp, off.data(), ss.data(), tran.data(), val.data());
+  std::vector<int> tag(m);
+  for (int i = 0; i < m; ++i) tag[i] = i;
+  if (n_symm > 1)
+    m = pdo_tran_symm(m, tran.data(), val.data(), tag.data(), rot, frame_a, frame_b, n_symm, mol_radius, max_rmsd,
+                      nullptr);
+  if (cluster_radius > 0) m = pdo_tran_clust(m, tran.data(), val.data(), tag.data(), cluster_radius, K, max_inp);
+  m = std::min(m, K);
+  for (int i = 0; i < m; ++i) {
+    const int t = tag[i];
+    offsets[i] = off[t];
+    for (int d = 0; d < 3; ++d) s[3 * i + d] = ss[3 * t + d];
+    values[i] = val[i];
+  }
+  return m;
+}
+
 int pdo_write_scan_record(const char* path, int append, const double* rot, int n,
                           const double* tran, const double* values) {
   FILE* f = std::fopen(path, append ? "ab" : "wb");

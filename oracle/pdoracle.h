@@ -112,6 +112,39 @@ int pdo_scan(const int* n, double h, int n_terms, const double* rec_spectra,
              int n_threads, int* n_out, int* offsets, int* s, double* values,
              double* corr_out);
 
+/* ---- post-filters of a rotation's translations --------------------------- */
+/* A rigid transformation x -> m x + v is 12 doubles: m (3x3 row-major), v (3). */
+
+/* TranSymm::process (docking_fft.hpp:146-230) with SymmetryCheckerCn
+ * (Geom/symmetry.hpp:17-62) and ToOriginalFrameTransformer
+ * (docking_molstruct.hpp:31-51): keeps, in order, the translations whose pose
+ * frame_a * (Translation(tran) * rot) * frame_b applied n_symm times moves the
+ * points radius*e1, radius*e2, radius*e3 and 0 by an RMSD below max_rmsd.
+ * Compacts tran (n x 3), values and tag (n, carried along) in place; returns
+ * the number kept.  rmsd (optional, n) receives every input's RMSD. */
+int pdo_tran_symm(int n, double* tran, double* values, int* tag, const double* rot,
+                  const double* frame_a, const double* frame_b, int n_symm, double mol_radius,
+                  double max_rmsd, double* rmsd);
+
+/* TranClust::process (docking_fft.hpp:30-140) with ClusterMatrix::cluster and
+ * getClusters (Geom/cluster.hpp:120-420, M_hash = 3): leaders of a greedy
+ * clustering with radius sqrt(3) * cluster_radius, visited in ascending order
+ * of -value; the leaders, in that order, replace the first entries of tran /
+ * values / tag.  Returns min(leaders, cap, max_out). */
+int pdo_tran_clust(int n, double* tran, double* values, int* tag, double cluster_radius, int max_out,
+                   int cap);
+
+/* One rotation's selection and post-processing chain of Worker::findTranslationalMinima
+ * + scanRotation (docking_app.hpp:417-466): the max_inp best translations below
+ * max_val (CorrelationProcessor::selectFromFFT), then TranSymm when n_symm > 1,
+ * then TranClust when cluster_radius > 0, then the first K
+ * (getTranValuesFilled).  Writes offsets / s / values of the output; returns
+ * their number.  rot (9), frames (12 each) as for pdo_tran_symm. */
+int pdo_select_post(const int* n, double h, const double* grid, int max_inp, double max_val, int K,
+                    const double* rot, const double* frame_a, const double* frame_b, int n_symm,
+                    double mol_radius, double max_rmsd, double cluster_radius, int* offsets, int* s,
+                    double* values);
+
 /* RotFftScanIO_Bin (docking_io_bin.hpp:21-76). */
 int pdo_write_scan_record(const char* path, int append, const double* rot,
                           int n, const double* tran, const double* values);

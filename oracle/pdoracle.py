@@ -50,6 +50,9 @@ def load() -> C.CDLL:
             "pdo_select": ([ip, d, dp, i, d, i, ip, ip, dp, dp], i),
             "pdo_direct_energy": ([ip, i, dp, i, ip, dp, ip], d),
             "pdo_scan": ([ip, d, i, dp, i, dp, dp, i, dp, i, d, i, i, ip, ip, ip, dp, dp], i),
+            "pdo_tran_symm": ([i, dp, dp, ip, dp, dp, dp, i, d, d, dp], i),
+            "pdo_tran_clust": ([i, dp, dp, ip, d, i, i], i),
+            "pdo_select_post": ([ip, d, dp, i, d, i, dp, dp, dp, i, d, d, d, ip, ip, dp], i),
             "pdo_write_scan_record": ([s, i, dp, i, dp, dp], i),
             "pdo_read_scan_records": ([s, i, i, dp, ip, dp, dp], i),
         }
@@ -268,6 +271,52 @@ def scan(n, h, rec_spectra, lig_pos, w, rots, K, max_inp=None, max_val=1e16, n_t
     if want_grids:
         return nout, off, s, val, grids
     return nout, off, s, val
+
+
+IDENTITY12 = np.array([1.0, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0])
+
+
+def tran_symm(tran, values, rot, frame_a=IDENTITY12, frame_b=IDENTITY12, n_symm=2, mol_radius=10.0,
+              max_rmsd=8.0):
+    """TranSymm on translations (n x 3) and values in list order; returns
+    (kept indices, rmsd of every input)."""
+    t = np.array(tran, np.float64).reshape(-1, 3).copy()
+    v = np.array(values, np.float64).copy()
+    tag = np.arange(len(v), dtype=np.int32)
+    r = np.ascontiguousarray(rot, np.float64).reshape(9)
+    fa = np.ascontiguousarray(frame_a, np.float64).reshape(12)
+    fb = np.ascontiguousarray(frame_b, np.float64).reshape(12)
+    rmsd = np.empty(len(v))
+    k = load().pdo_tran_symm(len(v), _d(t), _d(v), _i(tag), _d(r), _d(fa), _d(fb), int(n_symm), float(mol_radius),
+                             float(max_rmsd), _d(rmsd))
+    return tag[:k].copy(), rmsd
+
+
+def tran_clust(tran, values, cluster_radius, max_out, cap=None):
+    """TranClust on translations (n x 3) and values; returns the output's input indices."""
+    t = np.array(tran, np.float64).reshape(-1, 3).copy()
+    v = np.array(values, np.float64).copy()
+    tag = np.arange(len(v), dtype=np.int32)
+    k = load().pdo_tran_clust(len(v), _d(t), _d(v), _i(tag), float(cluster_radius), int(max_out),
+                              int(cap if cap is not None else len(v)))
+    return tag[:k].copy()
+
+
+def select_post(grid, h, max_inp, max_val, K, rot=None, frame_a=IDENTITY12, frame_b=IDENTITY12, n_symm=0,
+                mol_radius=0.0, max_rmsd=8.0, cluster_radius=0.0):
+    """One rotation's selection + TranSymm + TranClust + first K; returns (offsets, s, values)."""
+    g = np.ascontiguousarray(grid, np.float64)
+    n = _n(g.shape)
+    r = np.ascontiguousarray(np.eye(3) if rot is None else rot, np.float64).reshape(9)
+    fa = np.ascontiguousarray(frame_a, np.float64).reshape(12)
+    fb = np.ascontiguousarray(frame_b, np.float64).reshape(12)
+    off = np.empty(K, np.int32)
+    s = np.empty((K, 3), np.int32)
+    val = np.empty(K)
+    k = load().pdo_select_post(_i(n), float(h), _d(g), int(max_inp), float(max_val), int(K), _d(r), _d(fa), _d(fb),
+                               int(n_symm), float(mol_radius), float(max_rmsd), float(cluster_radius), _i(off),
+                               _i(s), _d(val))
+    return off[:k], s[:k], val[:k]
 
 
 def write_scan_record(path, rot, tran, values, append=True):

@@ -20,6 +20,7 @@
 #include "../../include/pdfft.h"
 #include "scan_fast.hpp"
 #include "scan_kernels.hpp"
+#include "scan_post.hpp"
 #include "scan_split.hpp"
 
 namespace pd_detail {
@@ -303,6 +304,7 @@ struct pd_scan {
     float* cval = nullptr;
     int* coff = nullptr;
     unsigned* thr = nullptr;  // per-rotation selection threshold of a batch
+    float* grid = nullptr;    // the batch's real correlation grids (post-filters, candidate lists)
     hipStream_t s = nullptr;
     hipEvent_t done = nullptr;
   };
@@ -323,6 +325,11 @@ struct pd_scan {
   std::vector<Pending> pending;
   double tot_ms[5] = {0, 0, 0, 0, 0};
   long long launches[5] = {0, 0, 0, 0, 0};
+  // Post-filters (pd_scan_set_postfilter): when on, each batch selects
+  // max_n_trans_inp candidates from its grids and filters them (k_select_post).
+  bool post = false;
+  PostGeom pg{};
+  bool grids = false;  // the lanes carry grid buffers
   // The graph whose nodes name this scan's buffers, if any: while it exists the
   // buffers must not move (set_batch, set_ligand and destroy are refused).
   pd_graph* graph = nullptr;
@@ -338,6 +345,7 @@ void free_batch(pd_scan* h) {
     hipFree(ln.cval); ln.cval = nullptr;
     hipFree(ln.coff); ln.coff = nullptr;
     hipFree(ln.thr); ln.thr = nullptr;
+    hipFree(ln.grid); ln.grid = nullptr;
   }
 }
 
@@ -355,7 +363,17 @@ pd_status alloc_batch(pd_scan* h) {
     PD_HIP(hipMalloc(&ln.cval, sizeof(float) * B * g.n0 * h->K));
     PD_HIP(hipMalloc(&ln.coff, sizeof(int) * B * g.n0 * h->K));
     PD_HIP(hipMalloc(&ln.thr, sizeof(unsigned) * B));
+    if (h->grids) PD_HIP(hipMalloc(&ln.grid, sizeof(float) * B * (size_t)h->N));
   }
+  return PD_OK;
+}
+
+// Grid buffers for the post-filter and candidate paths, added to the lanes.
+pd_status ensure_grids(pd_scan* h) {
+  if (h->grids) return PD_OK;
+  h->grids = true;
+  if (!h->ligand_ready) return PD_OK;  // alloc_batch adds them with the ligand
+  for (auto& ln : h->lane) PD_HIP(hipMalloc(&ln.grid, sizeof(float) * (size_t)h->B * (size_t)h->N));
   return PD_OK;
 }
 
@@ -608,12 +626,24 @@ pd_status launch_fft_path(pd_scan* h, const pd_scan::Lane& ln, int nb, const dou
 
 // One batch of nb rotations starting at rotation b0 of the call: the FFT path
 // and the per-rotation merge into the call's outputs.
+// Output of a batch: the first K peaks (merge of the per-plane lists, or the
+// post-filters' output), or the whole candidate list.
+enum class BatchOut { kPeaks, kCandidates };
+
 pd_status launch_batch(pd_scan* h, const pd_scan::Lane& ln, int b0, int nb, const double* d_rots, int rot0,
-                       PeakOut* d_peaks, int* d_nout, Launcher& L) {
+                       PeakOut* d_peaks, int* d_nout, Launcher& L, BatchOut what = BatchOut::kPeaks) {
   const ScanGeom& g = h->g;
-  pd_status st = launch_fft_path(h, ln, nb, d_rots + 9 * (size_t)b0, nullptr, L);
+  const bool select = h->post || what == BatchOut::kCandidates;
+  pd_status st = launch_fft_path(h, ln, nb, d_rots + 9 * (size_t)b0, select ? ln.grid : nullptr, L);
   if (st != PD_OK) return st;
-  {
+  if (select) {
+    Timer tm(h, 4, L.s);
+    PostGeom pg = h->pg;
+    pg.candidates = what == BatchOut::kCandidates;
+    const int stride = pg.candidates ? pg.k_inp : h->K;
+    L(k_select_post, dim3(nb), dim3(kSelThreads), 0, g, pg, ln.grid, d_rots + 9 * (size_t)b0, rot0 + b0,
+      d_peaks + (size_t)b0 * stride, d_nout + b0);
+  } else {
     Timer tm(h, 4, L.s);
     if (g.n0 <= 64 * kMergePlanes)
       L(k_merge_wave, dim3(nb), dim3(256), (size_t)g.n0 * h->K * 8, g, h->K, ln.cval, ln.coff,
@@ -645,7 +675,7 @@ int lanes_for(const pd_scan* h, int nbatch) {
 // Eager scan on the handle's stream: batches go round the lanes' streams,
 // which fork from and join back into the handle's stream.
 pd_status run_batches(pd_scan* h, int n_rot, const double* d_rots, int rot0, PeakOut* d_peaks,
-                      int* d_nout) {
+                      int* d_nout, BatchOut what = BatchOut::kPeaks) {
   pd_status st = check_run(h, n_rot);
   if (st != PD_OK) return st;
   const int nbatch = (n_rot + h->B - 1) / h->B;
@@ -662,7 +692,7 @@ pd_status run_batches(pd_scan* h, int n_rot, const double* d_rots, int rot0, Pea
     Launcher L;
     L.s = ln.s;
     const int b0 = i * h->B;
-    st = launch_batch(h, ln, b0, std::min(h->B, n_rot - b0), d_rots, rot0, d_peaks, d_nout, L);
+    st = launch_batch(h, ln, b0, std::min(h->B, n_rot - b0), d_rots, rot0, d_peaks, d_nout, L, what);
     if (st != PD_OK) return st;
   }
   if (nl > 1)
@@ -881,6 +911,9 @@ pd_status pd_scan_create(const pd_scan_desc* desc, pd_scan** out) {
   g.il = 0;
   g.ilsh = 0;
   h->N = (long long)g.n0 * g.n1 * g.n2;
+  h->pg.K = d.max_n_trans;
+  h->pg.k_inp = d.max_n_trans_inp;
+  h->pg.max_val = d.max_val_corr;
   h->C = std::max(1, std::min(64, 4096 / g.n0));
   auto bad = [&](pd_status s, const std::string& m) {
     pd_scan_destroy(h);
@@ -1167,6 +1200,73 @@ pd_status pd_scan_set_batch(pd_scan* h, int batch) {
     }
   }
   return PD_OK;
+}
+
+pd_status pd_scan_set_postfilter(pd_scan* h, const pd_postfilter* pf) {
+  if (!h) return fail(PD_ERR_INVALID_ARGUMENT, "null handle");
+  if (h->graph) return fail(PD_ERR_STATE, "the scan belongs to a graph: its buffers cannot change");
+  const bool symm = pf && pf->n_multimer > 1, clust = pf && pf->cluster != 0;
+  if (!symm && !clust) {
+    h->post = false;
+    return PD_OK;
+  }
+  if (h->pg.k_inp > kCandMax)
+    return fail(PD_ERR_UNSUPPORTED, "post-filters take max_n_trans_inp <= " + std::to_string(kCandMax));
+  if (symm && !(pf->ligand_radius >= 0 && pf->max_rmsd_symm == pf->max_rmsd_symm))
+    return fail(PD_ERR_INVALID_ARGUMENT, "bad symmetry parameters");
+  if (clust && !(pf->cluster_radius > 0)) return fail(PD_ERR_INVALID_ARGUMENT, "cluster_radius must be > 0");
+  PD_HIP(hipSetDevice(h->d.device));
+  PD_HIP(hipStreamSynchronize(h->stream));
+  for (auto& ln : h->lane) PD_HIP(hipStreamSynchronize(ln.s));
+  pd_status st = ensure_grids(h);
+  if (st != PD_OK) return st;
+  PostGeom& pg = h->pg;
+  pg.n_symm = symm ? pf->n_multimer : 0;
+  pg.cluster = clust ? 1 : 0;
+  pg.mol_radius = symm ? pf->ligand_radius : 0;
+  pg.max_rmsd = symm ? pf->max_rmsd_symm : 0;
+  // the neighbour test of ClusterMatrix: dist2 <= cutoff * cutoff with
+  // cutoff = sqrt(3 r^2) (Geom/cluster.hpp:160-164, partpoints.hpp:551-584)
+  const double r2 = clust ? pf->cluster_radius * pf->cluster_radius : 0;
+  const double cutoff = std::sqrt(r2 * 3);
+  pg.clust_rad2 = cutoff * cutoff;
+  for (int i = 0; i < 12; ++i) {
+    pg.fa[i] = symm ? pf->frame_rec_inv[i] : (i % 4 == 0 && i < 9 ? 1 : 0);
+    pg.fb[i] = symm ? pf->frame_lig[i] : (i % 4 == 0 && i < 9 ? 1 : 0);
+  }
+  h->post = true;
+  return PD_OK;
+}
+
+pd_status pd_scan_candidates(pd_scan* h, int n_rot, const double* rots, int rot_index0, pd_peak* peaks,
+                             int* n_out) {
+  if (!h || n_rot < 0 || (n_rot > 0 && (!rots || !peaks || !n_out)))
+    return fail(PD_ERR_INVALID_ARGUMENT, "null argument");
+  if (h->pg.k_inp > kCandMax)
+    return fail(PD_ERR_UNSUPPORTED, "candidate lists take max_n_trans_inp <= " + std::to_string(kCandMax));
+  if (h->graph) return fail(PD_ERR_STATE, "the scan belongs to a graph: its buffers cannot change");
+  if (n_rot == 0) return PD_OK;
+  PD_HIP(hipSetDevice(h->d.device));
+  pd_status st = ensure_grids(h);
+  if (st != PD_OK) return st;
+  st = ensure_io(h, n_rot);
+  if (st != PD_OK) return st;
+  PeakOut* d_cand = nullptr;
+  const size_t nc = (size_t)n_rot * h->pg.k_inp;
+  PD_HIP(hipMalloc(&d_cand, sizeof(PeakOut) * nc));
+  PD_HIP(hipMemcpyAsync(h->d_rots, rots, sizeof(double) * 9 * (size_t)n_rot, hipMemcpyHostToDevice, h->stream));
+  st = run_batches(h, n_rot, h->d_rots, rot_index0, d_cand, h->d_nout, BatchOut::kCandidates);
+  if (st == PD_OK) {
+    if (hipMemcpyAsync(peaks, d_cand, sizeof(PeakOut) * nc, hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+        hipMemcpyAsync(n_out, h->d_nout, sizeof(int) * (size_t)n_rot, hipMemcpyDeviceToHost, h->stream) !=
+            hipSuccess)
+      st = fail(PD_ERR_DEVICE, "candidate copy failed");
+    else
+      st = pd_scan_sync(h);
+  }
+  hipStreamSynchronize(h->stream);
+  hipFree(d_cand);
+  return st;
 }
 
 pd_status pd_scan_run_device(pd_scan* h, int n_rot, const double* d_rots, int rot_index0,

@@ -428,11 +428,6 @@ void rot_scan(const Options& opt, MolForceParams& mfp) {
   if (pot != "ljComp" && pot != "ljAvg")
     throw Error("not_supported_error: Unknown 'potentialName' parameter value: " + pot);
   if (pot == "ljComp" && mfp.mix != 1) throw Error("ljComp requires mix == 1 (LJ_MIX_1)");
-  // Worker::init post-filters (docking_app.hpp:369-398)
-  if (opt.flag("doClusterTranslations", false))
-    throw Error("not_supported_error: doClusterTranslations (TranClust) is not available in this build");
-  if (opt.integer("nMultimer", 0) > 1)
-    throw Error("not_supported_error: nMultimer > 1 (TranSymm) is not available in this build");
   const double h = opt.num("gridStep", 1.5);
   const double cut = opt.num("cutOffFft");
   const int maxNTrans = opt.integer("maxNTrans", 10);
@@ -483,6 +478,28 @@ void rot_scan(const Options& opt, MolForceParams& mfp) {
     w = wl;
   }
   check(pd_scan_set_ligand(h_, lig.n(), lig.pos.data(), w.data()));
+  // Worker::init post-filters (docking_app.hpp:363-398): TranSymm with the
+  // original-frame transformer and half the ligand's diameter, TranClust
+  const bool clust = opt.flag("doClusterTranslations", false);
+  const int n_multimer = opt.integer("nMultimer", 0);
+  if (clust || n_multimer > 1) {
+    pd_postfilter pf{};
+    pf.n_multimer = n_multimer;
+    pf.max_rmsd_symm = opt.num("maxRmsdSymm", 8.0);
+    pf.ligand_radius = fr.lig_size / 2;
+    const RT a = inverse(fr.tR);
+    for (int i = 0; i < 9; ++i) {
+      pf.frame_rec_inv[i] = a.m[i];
+      pf.frame_lig[i] = fr.tL.m[i];
+    }
+    for (int i = 0; i < 3; ++i) {
+      pf.frame_rec_inv[9 + i] = a.v[i];
+      pf.frame_lig[9 + i] = fr.tL.v[i];
+    }
+    pf.cluster = clust ? 1 : 0;
+    pf.cluster_radius = opt.num("clusterRadiusTrans", 5.0);
+    check(pd_scan_set_postfilter(h_, &pf));
+  }
 
   const int nr = end - start;
   std::vector<double> R(9 * (size_t)nr);

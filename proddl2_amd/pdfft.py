@@ -36,7 +36,7 @@ EXPORTS = (
     "pd_scan_kernel_stats", "pd_scan_stream", "pd_scan_destroy", "pd_last_error",
     "pd_find_best_size", "pd_ljcomp_params", "pd_scan_set_batch", "pd_graph_create", "pd_graph_launch",
     "pd_graph_sync", "pd_graph_stream", "pd_graph_destroy", "pd_ljavg_params",
-    "pd_scan_rescore64_noalloc",
+    "pd_scan_rescore64_noalloc", "pd_scan_set_postfilter", "pd_scan_candidates",
 )
 
 KERNEL_NAMES = ("k_bin", "k_lig_plane_fwd", "k_col", "k_plane_inv_topk", "k_merge")
@@ -68,6 +68,21 @@ SCAN_SPLIT_PLANES = 1
 SCAN_ONE_LANE = 2
 SCAN_SPLIT_INVERSE = 4
 SCAN_SPLIT_FORWARD = 8
+
+
+class PostFilter(C.Structure):
+    _fields_ = [
+        ("n_multimer", C.c_int),
+        ("max_rmsd_symm", C.c_double),
+        ("ligand_radius", C.c_double),
+        ("frame_rec_inv", C.c_double * 12),
+        ("frame_lig", C.c_double * 12),
+        ("cluster", C.c_int),
+        ("cluster_radius", C.c_double),
+    ]
+
+
+IDENTITY12 = (1.0, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0)
 
 
 class Term(C.Structure):
@@ -131,6 +146,8 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     lib.pd_ljcomp_params.argtypes = [i, dp, dp, dp, i, dp, dp, dp, dp, dp, dp]
     lib.pd_ljavg_params.argtypes = [i, dp, dp, dp, i, dp, dp, i, d, dp, dp]
     lib.pd_scan_set_batch.argtypes = [P, i]
+    lib.pd_scan_set_postfilter.argtypes = [P, C.POINTER(PostFilter)]
+    lib.pd_scan_candidates.argtypes = [P, i, dp, i, P, ip]
     PP = C.POINTER(P)
     lib.pd_graph_create.argtypes = [i, PP, ip, PP, PP, PP, ip, i, PP]
     lib.pd_graph_launch.argtypes = [P]
@@ -214,6 +231,7 @@ class Scan:
         self.n = tuple(int(x) for x in n)
         self.n_terms = int(n_terms)
         self.K = int(max_n_trans)
+        self.k_inp = int(d.max_n_trans_inp)
         self.grid_step = float(grid_step)
 
     def close(self) -> None:
@@ -316,6 +334,31 @@ class Scan:
 
     def set_batch(self, batch: int) -> None:
         _check(_lib.pd_scan_set_batch(self._h, int(batch)))
+
+    def set_postfilter(self, n_multimer=0, max_rmsd_symm=8.0, ligand_radius=0.0, frame_rec_inv=IDENTITY12,
+                       frame_lig=IDENTITY12, cluster=False, cluster_radius=5.0) -> None:
+        """TranSymm (n_multimer > 1) and / or TranClust (cluster) after the selection;
+        both off clears them."""
+        pf = PostFilter()
+        pf.n_multimer = int(n_multimer)
+        pf.max_rmsd_symm = float(max_rmsd_symm)
+        pf.ligand_radius = float(ligand_radius)
+        pf.frame_rec_inv[:] = [float(x) for x in np.asarray(frame_rec_inv, float).reshape(12)]
+        pf.frame_lig[:] = [float(x) for x in np.asarray(frame_lig, float).reshape(12)]
+        pf.cluster = int(bool(cluster))
+        pf.cluster_radius = float(cluster_radius)
+        _check(_lib.pd_scan_set_postfilter(self._h, C.byref(pf)))
+
+    def candidates(self, rots, rot_index0: int = 0):
+        """The max_n_trans_inp best translations of each rotation, before post-filters:
+        (peaks[n_rot, k_inp] structured, n_out[n_rot])."""
+        r = np.ascontiguousarray(rots, np.float64).reshape(-1, 9)
+        nr = r.shape[0]
+        peaks = np.zeros((nr, self.k_inp), PEAK_DTYPE)
+        nout = np.zeros(nr, np.int32)
+        _check(_lib.pd_scan_candidates(self._h, nr, _dp(r), int(rot_index0), peaks.ctypes.data_as(C.c_void_p),
+                                       _ip(nout)))
+        return peaks, nout
 
     @property
     def handle(self) -> int:

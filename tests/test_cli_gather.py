@@ -110,14 +110,11 @@ def test_gather_keeps_the_lowest_bounded(tmp_path):
 
 
 @pytest.mark.parametrize("option,value,what", [
-    ("doClusterTranslations", True, "TranClust"),
-    ("nMultimer", 2, "TranSymm"),
     ("potentialName", "ljFoo", "potentialName"),
 ])
 def test_rot_scan_refuses_what_it_cannot_do(tmp_path, option, value, what):
-    """Options the reference's Worker::init acts on (docking_app.hpp:56-67, 369-398)
-    that this build cannot honour end the task with not_supported_error before any
-    device work, instead of being ignored."""
+    """An option value the reference's App::init rejects (docking_app.hpp:56-67)
+    ends the task with not_supported_error before any device work."""
     if not os.path.exists(CLI):
         pytest.skip("CLI not built")
     rng = np.random.default_rng(4)

@@ -148,3 +148,46 @@ def test_rot_scan_ljavg_matches_oracle(tmp_path):
     rots, counts, tran, vals = pdoracle.read_scan_records(out)
     assert len(rots) == 12 and (counts == 6).all()
     _check_against_oracle(_ljavg_terms, rec, lig, p, len(rec), 1.5, 8.0, rots, tran[:, :6], vals[:, :6], 6)
+
+
+def test_rot_scan_postfilters_match_library(tmp_path):
+    """doClusterTranslations and nMultimer (Worker::init, docking_app.hpp:363-398)
+    through the CLI: the records equal the library's post-filtered scan of the
+    same rotations in the same frame (tR^-1 and tL of the frame, half the
+    ligand's diameter, clusterRadiusTrans, maxRmsdSymm)."""
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    rec, lig, p, mf, ang_path, a = _inputs(tmp_path, seed=14, n_ang=10)
+    h, cut, K, k_inp = 1.5, 8.0, 6, 400
+    opts = tmp_path / "opt.json"
+    opts.write_text(json.dumps({"gridStep": h, "cutOffFft": cut, "anglesFile": ang_path, "maxNTrans": K,
+                                "maxNTransInp": k_inp, "maxValCorr": 1e16, "doClusterTranslations": True,
+                                "clusterRadiusTrans": 4.0, "nMultimer": 2, "maxRmsdSymm": 30.0,
+                                "rescore64": False}))
+    out = _rot_scan(tmp_path, opts, mf, 0, 10)
+    rots, counts, tran, vals = pdoracle.read_scan_records(out)
+    nr = len(rec)
+    lig_mass = p["mass"][nr:]
+    rec_d, lig_d, n = _docking_frame(rec, lig, lig_mass, h, cut)
+    tR, tL = foreman.frames(rec, lig, lig_mass, SPIN)
+    ai = foreman.inverse(tR)
+    rep, att, wr, wa = pdfft.ljcomp_params(p["eps"][:nr], p["sigma"][:nr], p["alpha"][:nr], p["eps"][nr:],
+                                           p["sigma"][nr:])
+    d2 = ((lig_d[:, None, :] - lig_d[None, :, :]) ** 2).sum(-1)
+    with pdfft.Scan(n, h, 2, max_n_trans=K, max_n_trans_inp=k_inp, max_val_corr=1e16) as sc:
+        sc.project_receptor(rec_d, [(pdfft.FIELD_LJ_REP, cut, -1.0, rep), (pdfft.FIELD_LJ_ATT, cut, -1.0, att)])
+        sc.set_ligand(lig_d, np.stack([wr, wa]))
+        sc.set_postfilter(n_multimer=2, max_rmsd_symm=30.0, ligand_radius=np.sqrt(d2.max()) / 2,
+                          frame_rec_inv=np.concatenate([ai[0].reshape(9), ai[1]]),
+                          frame_lig=np.concatenate([tL[0].reshape(9), tL[1]]), cluster=True, cluster_radius=4.0)
+        peaks, nout = sc.run(rots.reshape(-1, 9))
+    same = 0
+    for r in range(len(rots)):
+        if counts[r] != nout[r]:
+            continue
+        k = int(nout[r])
+        if np.array_equal(tran[r, :k], h * peaks["s"][r, :k]) and np.array_equal(vals[r, :k], peaks["value"][r, :k]):
+            same += 1
+    print(f"  {same} of {len(rots)} rotations identical; outputs per rotation {list(counts)}", flush=True)
+    assert same >= len(rots) - 1
+    assert 0 < counts.sum() < K * len(rots) + 1
