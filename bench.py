@@ -38,25 +38,30 @@ def algorithmic_bytes(n, T, B):
     }, (2 * T + 2) * S + T * S / B
 
 
-def measured_traffic(workload, batch, kernel):
-    """HBM bytes per launch of ``kernel`` from the committed rocprofv3 counter summary.
+FP32_VALU_PEAK_TFLOPS = 78.6  # MI355X FP32 vector without packed math (FMA = 2 flops)
 
-    profiles/scan_<workload>_counters.md holds FETCH_SIZE/WRITE_SIZE per dispatch,
-    scaled by what the same counters read for a 1 GiB device copy; its JSON block
-    gives the calibrated bytes per launch.  None when no summary matches this
-    workload and batch size.
-    """
+
+def counter_summary(workload, batch):
+    """The JSON block of the committed rocprofv3 counter summary
+    (profiles/scan_<workload>_counters.md) when it matches this workload and
+    batch size, else None.  It holds, per kernel and launch, the HBM bytes
+    (FETCH_SIZE / WRITE_SIZE scaled by what the same counters read for a 1 GiB
+    device copy) and the executed FP32 flops (64 x (SQ_INSTS_VALU_ADD_F32 +
+    SQ_INSTS_VALU_MUL_F32 + 2 SQ_INSTS_VALU_FMA_F32))."""
     path = os.path.join(ROOT, "profiles", f"scan_{workload.replace('-', '')}_counters.md")
     if not os.path.exists(path):
         return None
-    text = open(path).read()
-    m = re.search(r"```json\n(.*?)\n```", text, re.S)
+    m = re.search(r"```json\n(.*?)\n```", open(path).read(), re.S)
     if not m:
         return None
     rec = json.loads(m.group(1))
-    if rec.get("batch") != batch:
-        return None
-    return rec.get("traffic_bytes_per_launch", {}).get(kernel)
+    return rec if rec.get("batch") == batch else None
+
+
+def measured_traffic(workload, batch, kernel):
+    """Calibrated HBM bytes per launch of ``kernel`` from the counter summary, or None."""
+    rec = counter_summary(workload, batch)
+    return None if rec is None else rec.get("traffic_bytes_per_launch", {}).get(kernel)
 
 
 def pruned_search(sc, cx, d_rots, ex_peaks, t_exhaustive_s, log):
@@ -257,6 +262,20 @@ def main() -> int:
     avg_s = ms_dom / launches_dom / 1e3
     bytes_per_launch = per_rot[dom] * n_rot / launches_dom
     achieved = bytes_per_launch / avg_s / 1e9
+    traffic = measured_traffic(args.workload, args.batch, dom)
+    # the bytes the pruned dataflow must move: the ligand planes of the box only
+    # (pp of N0 x planes per term), the receptor spectra once per batch, S1
+    # written and read once
+    S = 8 * n[0] * n[1] * (n[2] // 2 + 1)
+    _, _, pp = sc.ligand_box()
+    pruned_bytes = T * S * pp / n[0] + T * S / args.batch + 2 * S
+    summ = counter_summary(args.workload, args.batch) or {}
+    executed = {}
+    for kname, fl in (summ.get("fp32_flops_per_launch") or {}).items():
+        if kname in stats and stats[kname][1]:
+            t_launch = stats[kname][0] / stats[kname][1] / 1e3
+            tf = fl / t_launch / 1e12
+            executed[kname] = {"tflops": tf, "frac": tf / FP32_VALU_PEAK_TFLOPS, "flops_per_launch": fl}
     log(f"kernels: " + ", ".join(f"{k} {v['ms_total']:.1f} ms/{v['launches']}" for k, v in kern.items()))
 
     # FP64 rescoring of the global top poses (outside the timed region).
@@ -308,16 +327,26 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": measured_traffic(args.workload, args.batch, dom),
+            "traffic": traffic,
             "bytes_per_launch": bytes_per_launch,
             "avg_launch_us": avg_s * 1e6,
+            # the same launch priced by the bytes the counters saw moving
+            "traffic_GBs": traffic / avg_s / 1e9 if traffic else None,
+            "traffic_frac": traffic / avg_s / 1e9 / HBM_PEAK_GBS if traffic else None,
         },
         "pipeline_roofline": {
             "bytes_per_rotation": total_bytes,
             "achieved_GBs": rate * total_bytes / 1e9,
             "frac": rate * total_bytes / 1e9 / HBM_PEAK_GBS,
-            "fft_gflops": rate * (T + 1) * 5 * np.prod(n) * np.log2(np.prod(n)) / 1e9,
+            "fft_gflops_nominal": rate * (T + 1) * 5 * np.prod(n) * np.log2(np.prod(n)) / 1e9,
         },
+        "pruned_dataflow_roofline": {
+            "bytes_per_rotation": pruned_bytes,
+            "ligand_planes": pp,
+            "achieved_GBs": rate * pruned_bytes / 1e9,
+            "frac": rate * pruned_bytes / 1e9 / HBM_PEAK_GBS,
+        },
+        "executed_fp32": executed or None,
         "kernels": kern,
         "hbm_copy": hbm_copy,
         "receptor_precompute_ms": precompute_s * 1e3,

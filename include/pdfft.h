@@ -157,6 +157,12 @@ pd_status pd_scan_set_timing(struct pd_scan* h, int enable);
 pd_status pd_scan_kernel_stats(struct pd_scan* h, int k, double* total_ms, long long* launches);
 void* pd_scan_stream(struct pd_scan* h);
 
+/* The cells every rotated ligand atom can reach, [lo, lo + p) per axis (from
+ * the largest atom radius; no reference counterpart), and the x planes per
+ * (rotation, term) that the scan stores (p[0] rounded up to whole lane groups
+ * of the column pass). */
+pd_status pd_scan_ligand_box(struct pd_scan* h, int* lo, int* p, int* planes);
+
 /* Frees the handle.  PD_ERR_STATE (and the handle stays valid) while a graph
  * built from it exists. */
 pd_status pd_scan_destroy(struct pd_scan* h);

@@ -1396,6 +1396,17 @@ pd_status pd_scan_kernel_stats(pd_scan* h, int k, double* total_ms, long long* l
 
 void* pd_scan_stream(pd_scan* h) { return h ? (void*)h->stream : nullptr; }
 
+pd_status pd_scan_ligand_box(pd_scan* h, int* lo, int* p, int* planes) {
+  if (!h || !lo || !p || !planes) return fail(PD_ERR_INVALID_ARGUMENT, "null argument");
+  if (!h->ligand_ready) return fail(PD_ERR_STATE, "ligand not set");
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = h->g.lo[a];
+    p[a] = h->g.p[a];
+  }
+  *planes = h->g.pp;
+  return PD_OK;
+}
+
 }  // extern "C"
 
 // ---------------------------------------------------------------------------

@@ -36,7 +36,7 @@ EXPORTS = (
     "pd_scan_kernel_stats", "pd_scan_stream", "pd_scan_destroy", "pd_last_error",
     "pd_find_best_size", "pd_ljcomp_params", "pd_scan_set_batch", "pd_graph_create", "pd_graph_launch",
     "pd_graph_sync", "pd_graph_stream", "pd_graph_destroy", "pd_ljavg_params",
-    "pd_scan_rescore64_noalloc", "pd_scan_set_postfilter", "pd_scan_candidates",
+    "pd_scan_rescore64_noalloc", "pd_scan_set_postfilter", "pd_scan_candidates", "pd_scan_ligand_box",
 )
 
 KERNEL_NAMES = ("k_bin", "k_lig_plane_fwd", "k_col", "k_plane_inv_topk", "k_merge")
@@ -139,6 +139,7 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     lib.pd_scan_kernel_stats.argtypes = [P, i, dp, C.POINTER(C.c_longlong)]
     lib.pd_scan_stream.argtypes = [P]
     lib.pd_scan_stream.restype = P
+    lib.pd_scan_ligand_box.argtypes = [P, ip, ip, ip]
     lib.pd_scan_destroy.argtypes = [P]
     lib.pd_last_error.restype = C.c_char_p
     lib.pd_find_best_size.argtypes = [i]
@@ -328,6 +329,14 @@ class Scan:
             _check(_lib.pd_scan_kernel_stats(self._h, k, C.byref(ms), C.byref(n)))
             res[name] = (ms.value, n.value)
         return res
+
+    def ligand_box(self):
+        """(lo[3], p[3], stored x planes per rotation and term)."""
+        lo = np.zeros(3, np.int32)
+        p = np.zeros(3, np.int32)
+        planes = np.zeros(1, np.int32)
+        _check(_lib.pd_scan_ligand_box(self._h, _ip(lo), _ip(p), _ip(planes)))
+        return lo, p, int(planes[0])
 
     def stream(self) -> int:
         return _lib.pd_scan_stream(self._h) or 0
