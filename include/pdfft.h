@@ -248,6 +248,19 @@ pd_status pd_ljavg_params(int n_rec, const double* eps_r, const double* sigma_r,
                           int n_lig, const double* eps_l, const double* sigma_l, int mix, double cutoff,
                           double* params, double* w);
 
+/* Approximate minimum-volume bounding box of n points (pos: n x 3), the box
+ * MolStruct::moveIntoMinBox orients the receptor along (Geom::Bounding::Box
+ * with fixedDirections = false, Geom/bounding.hpp:135-183, docking_molstruct.hpp:
+ * 199-245).  Barequet & Har-Peled's grid search (grid 5) with local improvement,
+ * host code.  axes: 3 x 3, row k the unit direction of box axis k, a proper
+ * rotation, rows ordered by decreasing extent; lo, hi: the points' extent along
+ * each axis (lo[k] <= axes[k] . p <= hi[k]). */
+pd_status pd_min_volume_box(int n, const double* pos, double* axes, double* lo, double* hi);
+
+/* The two points farthest apart (Geom::Bounding::Diameter with eps = 0,
+ * Geom/bounding.hpp:33-97): exact, the first such pair i < j. */
+pd_status pd_diameter_pair(int n, const double* pos, int* i, int* j);
+
 #ifdef __cplusplus
 }
 #endif

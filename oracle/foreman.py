@@ -4,10 +4,10 @@ Restates, in FP64 numpy, what `proddl-dock-fft --task gather` computes from
 the rotation-scan files of the workers:
 
   - molecule frames (MolStruct::randomizeLigandOrientation / moveIntoMinBox,
-    include/PRODDL/docking_molstruct.hpp:199-266): the receptor moved by its box
-    transform (here the axis-aligned box centred at the origin), the ligand spun
-    about its centre of mass by startingSpin and moved so that the centre of its
-    diameter is at the origin;
+    include/PRODDL/docking_molstruct.hpp:199-266) for a given receptor box:
+    both molecules moved by the box transform (axes in rows, centre to the
+    origin), the ligand first spun about its centre of mass by startingSpin and
+    finally moved so that the centre of its diameter is at the origin;
   - Foreman::run (include/PRODDL/docking_app.hpp:174-220): every translation of
     every record becomes the rigid transformation Translation(tran) * Rotation(rot);
     the maxNRigidMatches lowest values below 1e16 are kept, ascending;
@@ -44,22 +44,51 @@ def inverse(a):
     return mt, mt @ (-a[1])
 
 
-def frames(rec_pos, lig_pos, lig_mass, spin):
-    """(tR, tL): original -> docking frame of receptor and ligand."""
+def _apply(t, x):
+    """x -> m x + v row by row, summed in the order the CLI sums (IEEE per op)."""
+    m, v = t
+    x = np.asarray(x, np.float64)
+    return np.stack([((m[i, 0] * x[:, 0] + m[i, 1] * x[:, 1]) + m[i, 2] * x[:, 2]) + v[i] for i in range(3)], axis=1)
+
+
+def docking_frame(rec_pos, lig_pos, lig_mass, spin, box, h=None, cut=None):
+    """The docking frame (MolStruct::moveIntoReferencePositions,
+    docking_molstruct.hpp:199-281) for a given receptor box (axes in rows, lo,
+    hi along them): the ligand spun about its centre of mass, both molecules
+    moved by x -> A x - (lo + hi) / 2, the ligand then centred on its diameter.
+    Returns (receptor, ligand, tR, tL, ligand size) and, given the grid step and
+    cutoff, the grid edges of the padded box (:232-243)."""
+    axes, blo, bhi = (np.asarray(a, np.float64) for a in box)
     rs = pdoracle.euler_to_matrix([spin, spin * 3.7877, spin * -4.513432])
     cm = (lig_mass[:, None] * lig_pos).sum(0) / lig_mass.sum()
-    spun = (lig_pos - cm) @ rs.T + cm
-    lo, hi = rec_pos.min(0), rec_pos.max(0)
-    c = 0.5 * (lo + hi)
-    d2 = ((spun[:, None, :] - spun[None, :, :]) ** 2).sum(-1)
+    spun = _apply((rs, np.zeros(3)), lig_pos - cm) + cm
+    tb = (axes.reshape(3, 3), -0.5 * (blo + bhi))
+    rec_d = _apply(tb, rec_pos)
+    lig_b = _apply(tb, spun)
+    d2 = ((lig_b[:, None, :] - lig_b[None, :, :]) ** 2).sum(-1)
     i, j = np.unravel_index(np.argmax(np.triu(d2, 1)), d2.shape)
-    mid = 0.5 * (spun[i] + spun[j])
-    tR = rt(v=-c)
-    tL = compose(rt(v=-mid), compose(compose(rt(v=cm), rt(m=rs)), rt(v=-cm)))
+    mid = 0.5 * (lig_b[i] + lig_b[j])
+    lig_d = lig_b - mid
+    tR = tb
+    spin_cm = compose(compose(rt(v=cm), rt(m=rs)), rt(v=-cm))
+    tL = compose(rt(v=-mid), compose(tb, spin_cm))
+    size = float(np.sqrt(d2[i, j]))
+    out = (rec_d, lig_d, tR, tL, size)
+    if h is None:
+        return out
+    pad = 2 * h + size / 2 + cut
+    half = 0.5 * (bhi - blo)
+    n = tuple(pdoracle.grid_size_from_box(-half[k] - pad, half[k] + pad, h) for k in range(3))
+    return out + (n,)
+
+
+def frames(rec_pos, lig_pos, lig_mass, spin, box):
+    """(tR, tL): original -> docking frame of receptor and ligand."""
+    _, _, tR, tL, _ = docking_frame(rec_pos, lig_pos, lig_mass, spin, box)
     return tR, tL
 
 
-def gather(scan_files, rec_pos, lig_pos, lig_mass, spin, max_matches):
+def gather(scan_files, rec_pos, lig_pos, lig_mass, spin, max_matches, box):
     """List of (value, rotation matrix, translation) in the original frame, ascending."""
     items = []
     for path in scan_files:
@@ -73,7 +102,7 @@ def gather(scan_files, rec_pos, lig_pos, lig_mass, spin, max_matches):
                     items.append((float(vals[r, k]), compose(rt(v=tran[r, k]), rt(m=rot))))
     items.sort(key=lambda x: x[0])
     items = items[:max_matches]
-    tR, tL = frames(rec_pos, lig_pos, lig_mass, spin)
+    tR, tL = frames(rec_pos, lig_pos, lig_mass, spin, box)
     tRi = inverse(tR)
     return [(v, *compose(compose(tRi, t), tL)) for v, t in items]
 

@@ -34,7 +34,7 @@ def _run(cmd: list[str]) -> None:
 
 def build_lib() -> str:
     os.makedirs(LIB_DIR, exist_ok=True)
-    srcs = [os.path.join(PKG, "csrc", f) for f in ("pdfft.hip", "pdprop.hip")]
+    srcs = [os.path.join(PKG, "csrc", f) for f in ("pdfft.hip", "pdprop.hip", "frame.cpp")]
     tmp = LIB + ".tmp"
     _run([HIPCC, *HIP_FLAGS, "-shared", "-o", tmp, *srcs])
     os.replace(tmp, LIB)
@@ -52,9 +52,16 @@ def build_cli() -> str:
     return CLI
 
 
+REFERENCE = os.environ.get("PD_REFERENCE", "/root/reference")
+
+
 def build_oracle() -> str:
     odir = os.path.join(ROOT, "oracle")
     _run(["make", "-C", odir, "-B"])
+    # the reference's own gdiam, a checker of the minimum-volume box: only where
+    # the reference sources are present
+    if os.path.exists(os.path.join(REFERENCE, "src", "PRODDL", "External", "Gdiam", "gdiam.cpp")):
+        _run(["make", "-C", odir, "-B", "ref", f"REF={REFERENCE}"])
     return os.path.join(odir, "_build", "libpdoracle.so")
 
 

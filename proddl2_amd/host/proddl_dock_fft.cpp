@@ -14,8 +14,10 @@
 // Differences from the reference, all forced by this image: molforce
 // parameters come in a small binary container (PDMF, written by
 // proddl2_amd.mfp) because HDF5 is absent; the receptor frame is its
-// axis-aligned bounding box centred at the origin (the reference uses the gdiam
-// approximate minimum-volume box, docking_molstruct.hpp:199-245).
+// approximate minimum-volume box comes from the library's own implementation
+// of the published method (pd_min_volume_box) instead of the vendored gdiam:
+// the same box, its axes possibly in another order and sign
+// (docking_molstruct.hpp:199-245).
 #include <algorithm>
 #include <array>
 #include <cctype>
@@ -306,6 +308,17 @@ RT inverse(const RT& a) {
   return r;
 }
 
+// x -> m x + v in place
+void apply(const RT& t, double* x) {
+  double y[3];
+  for (int i = 0; i < 3; ++i) {
+    double acc = 0;
+    for (int j = 0; j < 3; ++j) acc += t.m[3 * i + j] * x[j];
+    y[i] = acc + t.v[i];
+  }
+  std::memcpy(x, y, sizeof y);
+}
+
 RT translation(double x, double y, double z) {
   RT r;
   r.v[0] = x;
@@ -336,6 +349,10 @@ std::array<double, 3> matrix_to_euler(const Mat& m) {
   return {-a2, -a0, -a1};
 }
 
+void check(pd_status s) {
+  if (s != PD_OK) throw Error(std::string("libpdfft: ") + pd_last_error());
+}
+
 struct Frame {
   double box_lo[3], box_hi[3];
   double lig_size;
@@ -360,48 +377,47 @@ Frame prepare_frame(MolForceParams& p, const Options& opt) {
     apply(rs, x);
     for (int d = 0; d < 3; ++d) x[d] += cm[d];
   }
-  // receptor box: axis-aligned bounds, centred at the origin (both molecules move)
-  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
-  for (int a = 0; a < rec.n(); ++a)
-    for (int d = 0; d < 3; ++d) {
-      lo[d] = std::min(lo[d], rec.pos[3 * a + d]);
-      hi[d] = std::max(hi[d], rec.pos[3 * a + d]);
-    }
-  double c[3];
-  for (int d = 0; d < 3; ++d) c[d] = 0.5 * (lo[d] + hi[d]);
-  for (int a = 0; a < rec.n(); ++a)
-    for (int d = 0; d < 3; ++d) rec.pos[3 * a + d] -= c[d];
+  // receptor box: its approximate minimum-volume box (moveIntoMinBox,
+  // :199-245); both molecules move by x -> A x - A c, A the box axes in rows
+  // and c the box centre (Box::getTransformationToBoundCoordinates,
+  // Geom/bounding.hpp:196-206)
+  double axes[9], blo[3], bhi[3];
+  check(pd_min_volume_box(rec.n(), rec.pos.data(), axes, blo, bhi));
+  RT box;
+  for (int i = 0; i < 9; ++i) box.m[i] = axes[i];
+  // A c = the centre's coordinates along the axes, (lo + hi) / 2
+  for (int k = 0; k < 3; ++k) box.v[k] = -0.5 * (blo[k] + bhi[k]);
+  for (Molecule* m : {&rec, &lig})
+    for (int a = 0; a < m->n(); ++a) apply(box, &m->pos[3 * a]);
   // ligand: centre of its diameter at the origin (:215-226), exact diameter
   int bi = 0, bj = 0;
-  double best = -1;
-  for (int i = 0; i < lig.n(); ++i)
-    for (int j = i + 1; j < lig.n(); ++j) {
-      double s = 0;
-      for (int d = 0; d < 3; ++d) {
-        const double t = lig.pos[3 * i + d] - lig.pos[3 * j + d];
-        s += t * t;
-      }
-      if (s > best) { best = s; bi = i; bj = j; }
-    }
+  check(pd_diameter_pair(lig.n(), lig.pos.data(), &bi, &bj));
+  double best = 0;
+  for (int d = 0; d < 3; ++d) {
+    const double t = lig.pos[3 * bi + d] - lig.pos[3 * bj + d];
+    best += t * t;
+  }
   double mid[3];
   for (int d = 0; d < 3; ++d) mid[d] = 0.5 * (lig.pos[3 * bi + d] + lig.pos[3 * bj + d]);
   for (int j = 0; j < lig.n(); ++j)
     for (int d = 0; d < 3; ++d) lig.pos[3 * j + d] -= mid[d];
   Frame f;
   // trFromIni (docking_molstruct.hpp:199-266): receptor moved by the box
-  // transform; ligand spun about its centre of mass, then its diameter centre
-  // moved to the origin (the shared box shift cancels for the ligand)
-  f.tR = translation(-c[0], -c[1], -c[2]);
+  // transform; ligand spun about its centre of mass, moved by the box
+  // transform, then its diameter centre moved to the origin
+  f.tR = box;
   RT rspin;
   rspin.m = rs;
   const RT spin_cm = compose(compose(translation(cm[0], cm[1], cm[2]), rspin), translation(-cm[0], -cm[1], -cm[2]));
-  f.tL = compose(translation(-mid[0], -mid[1], -mid[2]), spin_cm);
+  f.tL = compose(translation(-mid[0], -mid[1], -mid[2]), compose(box, spin_cm));
   f.lig_size = std::sqrt(std::max(0.0, best));
   const double h = opt.num("gridStep", 1.5);
   const double pad = h * 2. + f.lig_size / 2. + opt.num("cutOffFft");  // :236
-  for (int d = 0; d < 3; ++d) {
-    f.box_lo[d] = lo[d] - c[d] - pad;
-    f.box_hi[d] = hi[d] - c[d] + pad;
+  // the receptor box in the docking frame: centred, half its extent per axis
+  for (int k = 0; k < 3; ++k) {
+    const double half = 0.5 * (bhi[k] - blo[k]);
+    f.box_lo[k] = -half - pad;
+    f.box_hi[k] = half + pad;
   }
   return f;
 }
@@ -415,9 +431,6 @@ int grid_edge(double lo, double hi, double h) {
   return n;
 }
 
-void check(pd_status s) {
-  if (s != PD_OK) throw Error(std::string("libpdfft: ") + pd_last_error());
-}
 
 // ---------------------------------------------------------------------------
 // rot-scan task: Worker::init + Worker::run

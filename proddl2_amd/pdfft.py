@@ -37,6 +37,7 @@ EXPORTS = (
     "pd_find_best_size", "pd_ljcomp_params", "pd_scan_set_batch", "pd_graph_create", "pd_graph_launch",
     "pd_graph_sync", "pd_graph_stream", "pd_graph_destroy", "pd_ljavg_params",
     "pd_scan_rescore64_noalloc", "pd_scan_set_postfilter", "pd_scan_candidates", "pd_scan_ligand_box",
+    "pd_min_volume_box", "pd_diameter_pair",
 )
 
 KERNEL_NAMES = ("k_bin", "k_lig_plane_fwd", "k_col", "k_plane_inv_topk", "k_merge")
@@ -207,6 +208,29 @@ def ljavg_params(eps_r, sigma_r, alpha_r, eps_l, sigma_l, mix=1, cutoff=9.0):
     _check(lib.pd_ljavg_params(len(er), _dp(er), _dp(sr), _dp(ar), len(el), _dp(el), _dp(sl), int(mix),
                                float(cutoff), _dp(par), _dp(w)))
     return par, w
+
+
+def min_volume_box(pos):
+    """Approximate minimum-volume box of the points (host code of the library;
+    Geom::Bounding::Box, Geom/bounding.hpp:135-183): (axes[3, 3] with the unit
+    box directions in rows, a proper rotation; lo[3]; hi[3])."""
+    lib = load()
+    p = np.ascontiguousarray(pos, np.float64).reshape(-1, 3)
+    axes = np.empty((3, 3))
+    lo = np.empty(3)
+    hi = np.empty(3)
+    _check(lib.pd_min_volume_box(len(p), _dp(p), _dp(axes), _dp(lo), _dp(hi)))
+    return axes, lo, hi
+
+
+def diameter_pair(pos):
+    """Indices (i, j), i < j, of the two points farthest apart (exact)."""
+    lib = load()
+    p = np.ascontiguousarray(pos, np.float64).reshape(-1, 3)
+    i = C.c_int()
+    j = C.c_int()
+    _check(lib.pd_diameter_pair(len(p), _dp(p), C.byref(i), C.byref(j)))
+    return i.value, j.value
 
 
 class Scan:

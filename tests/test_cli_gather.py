@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 from oracle import foreman, pdoracle
-from proddl2_amd import mfp, synth
+from proddl2_amd import pdfft, mfp, synth
 from proddl2_amd.build import CLI
 
 
@@ -58,15 +58,23 @@ def _run(tmp_path, rec, lig, spin, max_matches, seed=0):
     return foreman.read_iorigid(out), scans, masses
 
 
-def _molecules(seed=1):
+def _molecules(seed=1, cuboid=False):
+    """Receptor and ligand points; with cuboid, the receptor holds the corners of
+    [-15, 25] x [-10, 20] x [-5, 15], which are then its minimum box with the
+    coordinate axes as its axes."""
     rng = np.random.default_rng(seed)
-    return rng.uniform(-15, 25, (60, 3)), rng.uniform(-5, 9, (25, 3))
+    if not cuboid:
+        return rng.uniform(-15, 25, (60, 3)), rng.uniform(-5, 9, (25, 3))
+    lo, ext = np.array([-15.0, -10.0, -5.0]), np.array([40.0, 30.0, 20.0])
+    corners = np.array([[x, y, z] for x in (0, 1) for y in (0, 1) for z in (0, 1)])
+    rec = lo + np.concatenate([corners, rng.uniform(0, 1, (52, 3))]) * ext
+    return rec, rng.uniform(-5, 9, (25, 3))
 
 
 def test_gather_matches_oracle(tmp_path):
     rec, lig = _molecules()
     (val, ang, xyz), scans, masses = _run(tmp_path, rec, lig, spin=23.3734432, max_matches=40)
-    want = foreman.gather(scans, rec, lig, masses, 23.3734432, 40)
+    want = foreman.gather(scans, rec, lig, masses, 23.3734432, 40, pdfft.min_volume_box(rec))
     assert len(val) == len(want) == 40
     for i, (v, m, t) in enumerate(want):
         assert val[i] == np.float32(v)
@@ -78,11 +86,14 @@ def test_gather_matches_oracle(tmp_path):
 
 
 def test_gather_frame_transform_closed_form(tmp_path):
-    """startingSpin = 0: tL = Translation(-mid), tR = Translation(-c), so a match
-    t = (R, s) maps to rotation R and translation c + s - R mid."""
-    rec, lig = _molecules(2)
+    """startingSpin = 0 and a receptor whose minimum box is axis-parallel:
+    tL = Translation(-mid), tR = Translation(-c), so a match t = (R, s) maps to
+    rotation R and translation c + s - R mid."""
+    rec, lig = _molecules(2, cuboid=True)
+    axes, _, _ = pdfft.min_volume_box(rec)
+    assert np.abs(axes - np.eye(3)).max() < 1e-12
     (val, ang, xyz), scans, _ = _run(tmp_path, rec, lig, spin=0.0, max_matches=1000, seed=3)
-    c = 0.5 * (rec.min(0) + rec.max(0))
+    c = np.array([5.0, 5.0, 5.0])
     d2 = ((lig[:, None] - lig[None]) ** 2).sum(-1)
     i, j = np.unravel_index(np.argmax(d2), d2.shape)
     mid = 0.5 * (lig[i] + lig[j])

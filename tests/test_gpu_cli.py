@@ -38,20 +38,13 @@ def _inputs(tmp_path, n_rec=400, n_lig=90, seed=12, n_ang=40):
 
 def _docking_frame(rec, lig, lig_mass, h, cut):
     """The rot-scan frame (docking_molstruct.hpp:199-266 as the CLI states it):
-    receptor box centred at the origin, ligand spun about its centre of mass and
-    its diameter centred at the origin; grid edges from the padded box."""
-    tR, _ = foreman.frames(rec, lig, lig_mass, SPIN)
-    rec_d = rec + tR[1]
-    rs = pdoracle.euler_to_matrix([SPIN, SPIN * 3.7877, SPIN * -4.513432])
-    cm = (lig_mass[:, None] * lig).sum(0) / lig_mass.sum()
-    spun = (lig - cm) @ rs.T + cm
-    d2 = ((spun[:, None, :] - spun[None, :, :]) ** 2).sum(-1)
-    i, j = np.unravel_index(np.argmax(np.triu(d2, 1)), d2.shape)
-    lig_d = spun - 0.5 * (spun[i] + spun[j])
-    pad = 2 * h + np.sqrt(d2[i, j]) / 2 + cut
-    lo, hi = rec_d.min(0), rec_d.max(0)
-    n = tuple(pdoracle.grid_size_from_box(lo[d] - pad, hi[d] + pad, h) for d in range(3))
-    return rec_d, lig_d, n
+    both molecules moved into the receptor's minimum-volume box (the library's
+    pd_min_volume_box, checked against the reference's gdiam in
+    tests/test_frame.py), the ligand spun about its centre of mass and its
+    diameter centred at the origin; grid edges from the padded box."""
+    box = pdfft.min_volume_box(rec)
+    rec_d, lig_d, tR, tL, size, n = foreman.docking_frame(rec, lig, lig_mass, SPIN, box, h, cut)
+    return rec_d, lig_d, n, tR, tL
 
 
 def _rot_scan(tmp_path, opts, mf, lo, hi):
@@ -67,7 +60,7 @@ def _rot_scan(tmp_path, opts, mf, lo, hi):
 
 def _check_against_oracle(oracle_terms, rec, lig, p, n_rec, h, cut, rots, tran, vals, K):
     lig_mass = p["mass"][n_rec:]
-    rec_d, lig_d, n = _docking_frame(rec, lig, lig_mass, h, cut)
+    rec_d, lig_d, n, _, _ = _docking_frame(rec, lig, lig_mass, h, cut)
     rt, w = oracle_terms(p, n_rec)
     R = np.stack([pdoracle.project_receptor(n, h, rec_d, k, par, c, md) for (k, c, md, par) in rt])
     S = pdoracle.receptor_spectra(R)
@@ -128,7 +121,7 @@ def test_rot_scan_ranges_then_gather(tmp_path):
     subprocess.run([CLI, "--task", "gather", "--options", str(opts), "--molforce-params", mf,
                     "--fft-rot-scan-list", str(lst), "--fft-res", res], check=True, capture_output=True, timeout=120)
     val, ang_o, xyz = foreman.read_iorigid(res)
-    want = foreman.gather(outs, rec, lig, p["mass"][len(rec):], SPIN, 30)
+    want = foreman.gather(outs, rec, lig, p["mass"][len(rec):], SPIN, 30, pdfft.min_volume_box(rec))
     assert len(val) == 30
     for i, (v, m, t) in enumerate(want):
         assert val[i] == np.float32(v)
@@ -168,12 +161,12 @@ def test_rot_scan_postfilters_match_library(tmp_path):
     rots, counts, tran, vals = pdoracle.read_scan_records(out)
     nr = len(rec)
     lig_mass = p["mass"][nr:]
-    rec_d, lig_d, n = _docking_frame(rec, lig, lig_mass, h, cut)
-    tR, tL = foreman.frames(rec, lig, lig_mass, SPIN)
+    rec_d, lig_d, n, tR, tL = _docking_frame(rec, lig, lig_mass, h, cut)
     ai = foreman.inverse(tR)
     rep, att, wr, wa = pdfft.ljcomp_params(p["eps"][:nr], p["sigma"][:nr], p["alpha"][:nr], p["eps"][nr:],
                                            p["sigma"][nr:])
     d2 = ((lig_d[:, None, :] - lig_d[None, :, :]) ** 2).sum(-1)
+    print(f"  grid {n}", flush=True)
     with pdfft.Scan(n, h, 2, max_n_trans=K, max_n_trans_inp=k_inp, max_val_corr=1e16) as sc:
         sc.project_receptor(rec_d, [(pdfft.FIELD_LJ_REP, cut, -1.0, rep), (pdfft.FIELD_LJ_ATT, cut, -1.0, att)])
         sc.set_ligand(lig_d, np.stack([wr, wa]))
