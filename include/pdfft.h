@@ -229,6 +229,34 @@ pd_status pd_graph_sync(struct pd_graph* g);
 void* pd_graph_stream(struct pd_graph* g);
 void pd_graph_destroy(struct pd_graph* g);
 
+/* Launch choices of a scan for an autotuner (no counterpart in the reference,
+ * whose worker has one fixed loop).  0 in a field keeps the library's rule.
+ *   col_groups    rotation groups per column tile in the column pass: a tile's
+ *                 groups share its receptor spectra through one XCD's L2
+ *                 (1..batch; the library makes groups even-sized)
+ *   col_mz_extra  column-pass variant: this many variants above the smallest
+ *                 whose loads cover the ligand box (0..15; larger variants load
+ *                 more zero planes)
+ *   lig_pass      ligand pass: 1 the whole-plane kernel, 2 the row and column
+ *                 kernels (whole-plane grids only; PD_ERR_UNSUPPORTED where the
+ *                 choice does not exist for the grid)
+ *   lanes         batch lanes, the streams consecutive batches go round (1..3)
+ * Results do not depend on col_groups or lanes (same bits).  col_mz_extra and
+ * lig_pass can select other transform arithmetic (a first DFT stage that skips
+ * the zero planes, another order of the plane transforms): same FP32 bar, not
+ * always the same bits.  Changing lig_pass after pd_scan_set_ligand re-lays the
+ * ligand.  PD_ERR_STATE while a graph holds the scan. */
+typedef struct pd_tuning {
+  int col_groups;
+  int col_mz_extra;
+  int lig_pass;
+  int lanes;
+} pd_tuning;
+pd_status pd_scan_set_tuning(struct pd_scan* h, const pd_tuning* t); /* t null: every rule */
+/* The choices in effect for a launch of `batch` rotations (rules resolved;
+ * after pd_scan_set_ligand, since the column pass follows the ligand box). */
+pd_status pd_scan_get_tuning(struct pd_scan* h, int batch, pd_tuning* t);
+
 /* FFTW_Size::findBestSize (Math/fft.hpp:141-151); returns -1 if none. */
 int pd_find_best_size(int start);
 

@@ -279,6 +279,10 @@ struct pd_scan {
   bool inv_fused = false;            // split inverse as one kernel (N1 = N2 = 256)
   bool lig_fused = false;            // split ligand pass as one kernel (N1 = N2 = 256)
   bool lig_rows = false;             // whole-plane grid whose ligand pass runs as row + column kernels
+  bool rows_ok = false;              // the row + column ligand pass fits this whole-plane grid
+  bool rows_auto = false;            // the library's rule for lig_rows
+  pd_tuning tune{};                  // pd_scan_set_tuning (0: rule)
+  std::vector<double> lig_pos_h, lig_w_h;  // the ligand as set, to re-lay it when lig_pass changes
   bool split = false;  // plane passes as row + column kernels (scan_split.hpp)
   HostPlan hx, hy, hz;
   FftPlan px{}, py{}, pz{};
@@ -515,6 +519,55 @@ pd_status receptor_spectra(pd_scan* h) {
   return PD_OK;
 }
 
+// Launch shape of the column pass for a batch of nb rotations: the kernel
+// (specialised or generic), its tiles, rotation groups per tile (nyb) of bpb
+// rotations, and the specialised kernel's MZ variant.
+struct ColShape {
+  const FastCol* fc;
+  int ntile, nyb, bpb, v;
+};
+
+ColShape col_shape(const pd_scan* h, int nb) {
+  const ScanGeom& g = h->g;
+  // The specialised column kernel addresses a block group's ligand planes and
+  // outputs through 32-bit buffer offsets: cap the rotations per group.
+  const FastCol* fc = h->fcol;
+  const long long lig_bytes = (long long)g.T * g.pp * g.M * 8, out_bytes = (long long)g.n0 * g.M * 8;
+  const long long max_bpb = std::min(0xFFFFFFFFLL / lig_bytes, 0xFFFFFFFFLL / out_bytes);
+  if (max_bpb < 1) fc = nullptr;
+  const int C = fc ? fc->col_lines : h->C;
+  const int ntile = (g.M + C - 1) / C;
+  int nyb = std::max(1, std::min(nb, 4096 / std::max(1, ntile)));
+  if (fc) {
+    // About 1,024 blocks: the rotation groups of a tile run together on one
+    // XCD and share each read of the tile's receptor spectra through L2,
+    // and fewer, longer blocks leave CUs to the other lanes' plane passes.
+    // Measured with three lanes at 128^3 (260 tiles): 4 groups of 16
+    // rotations 91.4-91.7k rotations/s, 8 groups 90.2-91.5k, 16 groups
+    // 89.7-90.4k, 2 groups 89.6-90.2k; the column pass timed alone is
+    // fastest with 16 groups (158 against 190 ms per 30,000 rotations).
+    // 256^3 (2,064 tiles) gets 2 groups of 32, the fewest its 32-bit
+    // offsets allow (column pass 525 ms per 10,000 rotations against 607
+    // with 16 groups of 4).
+    nyb = std::max(1, std::min((nb + 1) / 2, (1024 + ntile / 2) / ntile));
+  }
+  if (h->tune.col_groups > 0) nyb = std::min(nb, h->tune.col_groups);
+  if (fc) nyb = std::max(nyb, (int)((nb + max_bpb - 1) / max_bpb));
+  int bpb = (nb + nyb - 1) / nyb;
+  // the fast column pass takes rotations in pairs: an even group size
+  // leaves no phantom rotation except at an odd batch end
+  if (fc && bpb % 2 && bpb < max_bpb) ++bpb;
+  nyb = (nb + bpb - 1) / bpb;
+  ColShape cs{fc, ntile, nyb, bpb, 0};
+  if (fc) {
+    const int mz = (g.pp + fc->lanes - 1) / fc->lanes;  // positions per lane that hold ligand planes
+    int v = 0;
+    while (v < 15 && fc->mz[v] < mz) ++v;
+    cs.v = std::min(15, v + h->tune.col_mz_extra);
+  }
+  return cs;
+}
+
 // The four FFT-path kernels of one batch of nb rotations (bin, ligand plane
 // pass, column pass, inverse plane pass + top-K).  grid_out (optional)
 // receives the scaled correlation grids.
@@ -556,40 +609,12 @@ pd_status launch_fft_path(pd_scan* h, const pd_scan::Lane& ln, int nb, const dou
   }
   {
     Timer tm(h, 2, stream);
-    // The specialised column kernel addresses a block group's ligand planes and
-    // outputs through 32-bit buffer offsets: cap the rotations per group.
-    const FastCol* fc = h->fcol;
-    const long long lig_bytes = (long long)g.T * g.pp * g.M * 8, out_bytes = (long long)g.n0 * g.M * 8;
-    const long long max_bpb = std::min(0xFFFFFFFFLL / lig_bytes, 0xFFFFFFFFLL / out_bytes);
-    if (max_bpb < 1) fc = nullptr;
-    const int C = fc ? fc->col_lines : h->C;
-    const int ntile = (g.M + C - 1) / C;
-    int nyb = std::max(1, std::min(nb, 4096 / std::max(1, ntile)));
-    if (fc) {
-      // About 1,024 blocks: the rotation groups of a tile run together on one
-      // XCD and share each read of the tile's receptor spectra through L2,
-      // and fewer, longer blocks leave CUs to the other lanes' plane passes.
-      // Measured with three lanes at 128^3 (260 tiles): 4 groups of 16
-      // rotations 91.4-91.7k rotations/s, 8 groups 90.2-91.5k, 16 groups
-      // 89.7-90.4k, 2 groups 89.6-90.2k; the column pass timed alone is
-      // fastest with 16 groups (158 against 190 ms per 30,000 rotations).
-      // 256^3 (2,064 tiles) gets 2 groups of 32, the fewest its 32-bit
-      // offsets allow (column pass 525 ms per 10,000 rotations against 607
-      // with 16 groups of 4).
-      nyb = std::max(1, std::min((nb + 1) / 2, (1024 + ntile / 2) / ntile));
-      nyb = std::max(nyb, (int)((nb + max_bpb - 1) / max_bpb));
-    }
-    int bpb = (nb + nyb - 1) / nyb;
-    // the fast column pass takes rotations in pairs: an even group size
-    // leaves no phantom rotation except at an odd batch end
-    if (fc && bpb % 2 && bpb < max_bpb) ++bpb;
-    nyb = (nb + bpb - 1) / bpb;
+    const ColShape cs = col_shape(h, nb);
+    const FastCol* fc = cs.fc;
+    const int ntile = cs.ntile, nyb = cs.nyb, bpb = cs.bpb;
     if (fc) {
       const int nblk = 8 * ((ntile + 7) / 8) * nyb;
-      const int mz = (g.pp + fc->lanes - 1) / fc->lanes;  // positions per lane that hold ligand planes
-      int v = 0;
-      while (v < 15 && fc->mz[v] < mz) ++v;
-      L(fc->col[v], dim3(nblk), dim3(kColThreads), 0, g, h->tw_x, ln.L1, h->Rlanes,
+      L(fc->col[cs.v], dim3(nblk), dim3(kColThreads), 0, g, h->tw_x, ln.L1, h->Rlanes,
                          ln.S1, bpb, nb, nyb);
     } else
       L(k_col, dim3(ntile, nyb), dim3(kColThreads), col_lds(h), g, h->px,
@@ -669,7 +694,7 @@ pd_status check_run(pd_scan* h, int n_rot) {
 // or for a single batch.
 int lanes_for(const pd_scan* h, int nbatch) {
   const bool single = h->timing || nbatch <= 1 || (h->d.flags & PD_SCAN_ONE_LANE) != 0;
-  return single ? 1 : pd_scan::kLanes;
+  return single ? 1 : (h->tune.lanes > 0 ? h->tune.lanes : pd_scan::kLanes);
 }
 
 // Eager scan on the handle's stream: batches go round the lanes' streams,
@@ -962,10 +987,11 @@ pd_status pd_scan_create(const pd_scan_desc* desc, pd_scan** out) {
   // CU: 144^3, 160^3) runs the ligand pass as the split row and column
   // kernels, which measured twice as fast there (160^3: 32.9 -> 15.4 us per
   // rotation); its inverse keeps the whole-plane kernel.
-  h->lig_rows = !h->split && 2 * plane_lds(h) > kMaxLds && plan_fits(h->hz, g.zrows, kSplitThreads) &&
-                plan_fits(h->hy, kSplitCols, kSplitThreads) && split_row_lds(h) <= kMaxLds &&
-                split_col_lds(h) <= kMaxLds;
-  if (h->lig_rows) {
+  h->rows_ok = !h->split && plan_fits(h->hz, g.zrows, kSplitThreads) && plan_fits(h->hy, kSplitCols, kSplitThreads) &&
+               split_row_lds(h) <= kMaxLds && split_col_lds(h) <= kMaxLds;
+  h->rows_auto = h->rows_ok && 2 * plane_lds(h) > kMaxLds;
+  h->lig_rows = h->rows_auto;
+  if (h->rows_ok) {
     if ((st = set_lds((const void*)k_lig_rows, split_row_lds(h))) != PD_OK ||
         (st = set_lds((const void*)k_cols_fft<-1>, split_col_lds(h))) != PD_OK)
       return bad(st, pd_detail::last_error());
@@ -1172,6 +1198,10 @@ pd_status pd_scan_set_ligand(pd_scan* h, int n_atoms, const double* pos, const d
   PD_HIP(hipMemcpy(h->w64, weights, sizeof(double) * nw, hipMemcpyHostToDevice));
   PD_HIP(hipMemcpy(h->w32, w32.data(), sizeof(float) * nw, hipMemcpyHostToDevice));
   h->n_lig = n_atoms;
+  if (h->lig_pos_h.data() != pos) {
+    h->lig_pos_h.assign(pos, pos + 3 * (size_t)n_atoms);
+    h->lig_w_h.assign(weights, weights + nw);
+  }
   pd_status st = alloc_batch(h);
   if (st != PD_OK) return st;
   if (h->receptor_ready && h->rhat_shift != g.pl) {
@@ -1199,6 +1229,44 @@ pd_status pd_scan_set_batch(pd_scan* h, int batch) {
       return st;
     }
   }
+  return PD_OK;
+}
+
+pd_status pd_scan_set_tuning(pd_scan* h, const pd_tuning* t) {
+  if (!h) return fail(PD_ERR_INVALID_ARGUMENT, "null handle");
+  if (h->graph) return fail(PD_ERR_STATE, "the scan belongs to a graph: its launches are fixed");
+  const pd_tuning z{};
+  const pd_tuning& u = t ? *t : z;
+  if (u.col_groups < 0 || u.col_mz_extra < 0 || u.col_mz_extra > 15 || u.lig_pass < 0 || u.lig_pass > 2 ||
+      u.lanes < 0 || u.lanes > pd_scan::kLanes)
+    return fail(PD_ERR_INVALID_ARGUMENT, "tuning value out of range");
+  if (u.lig_pass == 2 && !h->rows_ok)
+    return fail(PD_ERR_UNSUPPORTED, "this grid has no row and column ligand pass");
+  if (u.lig_pass == 1 && h->split) return fail(PD_ERR_UNSUPPORTED, "this grid has no whole-plane ligand pass");
+  PD_HIP(hipSetDevice(h->d.device));
+  PD_HIP(hipStreamSynchronize(h->stream));
+  for (auto& ln : h->lane) PD_HIP(hipStreamSynchronize(ln.s));
+  h->tune = u;
+  const bool rows = u.lig_pass == 0 ? h->rows_auto : u.lig_pass == 2;
+  if (!h->split && rows != h->lig_rows) {
+    h->lig_rows = rows;
+    // the ligand's L1 layout depends on the ligand pass: lay it out again
+    if (h->ligand_ready) {
+      const std::vector<double> pos = h->lig_pos_h, w = h->lig_w_h;
+      return pd_scan_set_ligand(h, (int)(pos.size() / 3), pos.data(), w.data());
+    }
+  }
+  return PD_OK;
+}
+
+pd_status pd_scan_get_tuning(pd_scan* h, int batch, pd_tuning* t) {
+  if (!h || !t || batch < 1) return fail(PD_ERR_INVALID_ARGUMENT, "bad tuning query");
+  if (!h->ligand_ready) return fail(PD_ERR_STATE, "the launch shape follows the ligand box: set the ligand first");
+  const ColShape cs = col_shape(h, batch);
+  t->col_groups = cs.nyb;
+  t->col_mz_extra = h->tune.col_mz_extra;
+  t->lig_pass = h->split ? 0 : (h->lig_rows ? 2 : 1);
+  t->lanes = lanes_for(h, 2);
   return PD_OK;
 }
 

@@ -37,7 +37,7 @@ EXPORTS = (
     "pd_find_best_size", "pd_ljcomp_params", "pd_scan_set_batch", "pd_graph_create", "pd_graph_launch",
     "pd_graph_sync", "pd_graph_stream", "pd_graph_destroy", "pd_ljavg_params",
     "pd_scan_rescore64_noalloc", "pd_scan_set_postfilter", "pd_scan_candidates", "pd_scan_ligand_box",
-    "pd_min_volume_box", "pd_diameter_pair",
+    "pd_min_volume_box", "pd_diameter_pair", "pd_scan_set_tuning", "pd_scan_get_tuning",
 )
 
 KERNEL_NAMES = ("k_bin", "k_lig_plane_fwd", "k_col", "k_plane_inv_topk", "k_merge")
@@ -84,6 +84,14 @@ class PostFilter(C.Structure):
 
 
 IDENTITY12 = (1.0, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0)
+
+
+class Tuning(C.Structure):
+    """pd_tuning: launch choices for an autotuner (0 keeps the library's rule)."""
+    _fields_ = [("col_groups", C.c_int), ("col_mz_extra", C.c_int), ("lig_pass", C.c_int), ("lanes", C.c_int)]
+
+    def as_dict(self):
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
 
 
 class Term(C.Structure):
@@ -256,6 +264,7 @@ class Scan:
         self.n = tuple(int(x) for x in n)
         self.n_terms = int(n_terms)
         self.K = int(max_n_trans)
+        self.batch = int(batch) or 64
         self.k_inp = int(d.max_n_trans_inp)
         self.grid_step = float(grid_step)
 
@@ -367,6 +376,18 @@ class Scan:
 
     def set_batch(self, batch: int) -> None:
         _check(_lib.pd_scan_set_batch(self._h, int(batch)))
+        self.batch = int(batch)
+
+    def set_tuning(self, col_groups=0, col_mz_extra=0, lig_pass=0, lanes=0) -> None:
+        """Launch choices (pd_scan_set_tuning); all 0: the library's rules."""
+        t = Tuning(int(col_groups), int(col_mz_extra), int(lig_pass), int(lanes))
+        _check(_lib.pd_scan_set_tuning(self._h, C.byref(t)))
+
+    def tuning(self, batch=None) -> dict:
+        """The choices in effect for a launch of `batch` rotations (default: the handle's batch)."""
+        t = Tuning()
+        _check(_lib.pd_scan_get_tuning(self._h, int(batch or self.batch), C.byref(t)))
+        return t.as_dict()
 
     def set_postfilter(self, n_multimer=0, max_rmsd_symm=8.0, ligand_radius=0.0, frame_rec_inv=IDENTITY12,
                        frame_lig=IDENTITY12, cluster=False, cluster_radius=5.0) -> None:
