@@ -42,6 +42,7 @@ constexpr int OFF_B3 = OFF_W3 + HID;
 constexpr int NPARAM = OFF_B3 + 1;  // 167953
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // Round to the nearest BF16 (ties to even), returned as FP32.
@@ -567,7 +568,7 @@ __global__ __launch_bounds__(256) void k_zscore_bwd(int n, const float* __restri
 // ---------------------------------------------------------------------------
 // GEMM on MFMA: C[m][n] = sum_k A(m, k) B(k, n), operands rounded to BF16,
 // FP32 accumulation.  A(m, k) = A[m * sam + k * sak], B(k, n) = B[k * sbk + n * sbn].
-// 64 x 64 tile per block (4 waves, 32 x 32 each = 2 x 2 MFMA 16x16x16 tiles).
+// 64 x 64 tile per block (4 waves, 32 x 32 each = 2 x 2 MFMA 16x16x32 tiles).
 // Epilogue 0: store; 1: + bias[n], ReLU; 2: keep where mask[m * ldm + n] > 0.
 // ---------------------------------------------------------------------------
 constexpr int GT = 64, GK = 32, GKP = GK + 8;
@@ -618,19 +619,21 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(int M, int N, int K, const fl
       Bs[n][k] = bf16bits(w);
     }
     __syncthreads();
-#pragma unroll
-    for (int kk = 0; kk < GK; kk += 16) {
-      bf16x4 af[2], bfr[2];
+    // one gfx950 16x16x32 BF16 MFMA per output tile and k tile: lane l holds
+    // row (or column) l % 16 and the 8 k values 8 (l / 16) .. 8 (l / 16) + 7
+    static_assert(GK == 32, "one 16x16x32 MFMA step per k tile");
+    {
+      bf16x8 af[2], bfr[2];
 #pragma unroll
       for (int a = 0; a < 2; ++a)
-        af[a] = *reinterpret_cast<const bf16x4*>(&As[wm + a * 16 + (lane & 15)][kk + 4 * (lane >> 4)]);
+        af[a] = *reinterpret_cast<const bf16x8*>(&As[wm + a * 16 + (lane & 15)][8 * (lane >> 4)]);
 #pragma unroll
       for (int b = 0; b < 2; ++b)
-        bfr[b] = *reinterpret_cast<const bf16x4*>(&Bs[wn + b * 16 + (lane & 15)][kk + 4 * (lane >> 4)]);
+        bfr[b] = *reinterpret_cast<const bf16x8*>(&Bs[wn + b * 16 + (lane & 15)][8 * (lane >> 4)]);
 #pragma unroll
       for (int a = 0; a < 2; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af[a], bfr[b], acc[a][b], 0, 0, 0);
+        for (int b = 0; b < 2; ++b) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a], bfr[b], acc[a][b], 0, 0, 0);
     }
     __syncthreads();
   }
