@@ -34,7 +34,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <new>
 #include <string>
+#include <system_error>
 #include <thread>
 #include <vector>
 
@@ -253,11 +255,17 @@ OBox min_volume_box(const Pts& p) {
   std::vector<OBox> cand(dirs.size());
   const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
   const size_t nt = std::min<size_t>(hw, dirs.size());
+  auto work = [&](size_t t) {
+    for (size_t k = t; k < dirs.size(); k += nt) cand[k] = improve(best_box_along(dirs[k], p), p, 10);
+  };
   std::vector<std::thread> pool;
-  for (size_t t = 0; t < nt; ++t)
-    pool.emplace_back([&, t] {
-      for (size_t k = t; k < dirs.size(); k += nt) cand[k] = improve(best_box_along(dirs[k], p), p, 10);
-    });
+  size_t started = 0;
+  try {
+    for (; started < nt; ++started) pool.emplace_back(work, started);
+  } catch (const std::system_error&) {
+    // no more threads: this thread takes the slices that did not start
+  }
+  for (size_t t = started; t < nt; ++t) work(t);
   for (auto& th : pool) th.join();
   OBox best = base;
   for (const OBox& c : cand)
@@ -300,8 +308,13 @@ bool finite(int n, const double* pos) {
 extern "C" pd_status pd_min_volume_box(int n, const double* pos, double* axes, double* lo, double* hi) {
   if (n < 1 || !pos || !axes || !lo || !hi) return invalid("pd_min_volume_box: null pointer or no points");
   if (!finite(n, pos)) return invalid("pd_min_volume_box: non-finite coordinate");
-  const Pts p = points(n, pos);
-  const OBox b = canonical(min_volume_box(p), p);
+  OBox b;
+  try {
+    const Pts p = points(n, pos);
+    b = canonical(min_volume_box(p), p);
+  } catch (const std::bad_alloc&) {
+    return invalid("pd_min_volume_box: out of host memory");
+  }
   for (int k = 0; k < 3; ++k) {
     axes[3 * k] = b.ax[k].x;
     axes[3 * k + 1] = b.ax[k].y;
@@ -315,6 +328,10 @@ extern "C" pd_status pd_min_volume_box(int n, const double* pos, double* axes, d
 extern "C" pd_status pd_diameter_pair(int n, const double* pos, int* i, int* j) {
   if (n < 1 || !pos || !i || !j) return invalid("pd_diameter_pair: null pointer or no points");
   if (!finite(n, pos)) return invalid("pd_diameter_pair: non-finite coordinate");
-  diameter(points(n, pos), *i, *j);
+  try {
+    diameter(points(n, pos), *i, *j);
+  } catch (const std::bad_alloc&) {
+    return invalid("pd_diameter_pair: out of host memory");
+  }
   return PD_OK;
 }
