@@ -74,6 +74,9 @@ constexpr uint64_t kNoKey = ~0ull;
 
 __device__ __forceinline__ uint64_t pd_key(float v, int off) {
   uint32_t b = __float_as_uint(v);
+  // -0.0 and +0.0 are equal to the reference's operator<: one key, so that
+  // among them the earlier cell wins as among any equal values
+  if (b == 0x80000000u) b = 0u;
   b = (b & 0x80000000u) ? ~b : (b | 0x80000000u);
   return ((uint64_t)b << 32) | (uint32_t)off;
 }

@@ -65,6 +65,11 @@ def test_queue_tie_rule_and_row_major_order(oracle):
     g = np.zeros((3, 4, 5))
     off, s, tran, v = oracle.select(g, 1.0, 3, 1.0, 3)
     assert sorted(off) == [0, 1, 2]
+    # -0.0 and +0.0 are equal under operator<: the first cells still win
+    g = np.zeros((3, 4, 5))
+    g.reshape(-1)[[1, 4, 7, 30, 59]] = -0.0
+    off, s, tran, v = oracle.select(g, 1.0, 3, 1.0, 3)
+    assert sorted(off) == [0, 1, 2]
     # maxValCorr is a strict bound (queue.hpp:225-226)
     g = np.arange(60, dtype=float).reshape(3, 4, 5)
     off, s, tran, v = oracle.select(g, 1.0, 100, 10.0, 100)
