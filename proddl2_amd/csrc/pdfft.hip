@@ -1246,6 +1246,8 @@ pd_status pd_scan_set_tuning(pd_scan* h, const pd_tuning* t) {
   PD_HIP(hipSetDevice(h->d.device));
   PD_HIP(hipStreamSynchronize(h->stream));
   for (auto& ln : h->lane) PD_HIP(hipStreamSynchronize(ln.s));
+  const pd_tuning old = h->tune;
+  const bool old_rows = h->lig_rows;
   h->tune = u;
   const bool rows = u.lig_pass == 0 ? h->rows_auto : u.lig_pass == 2;
   if (!h->split && rows != h->lig_rows) {
@@ -1253,7 +1255,16 @@ pd_status pd_scan_set_tuning(pd_scan* h, const pd_tuning* t) {
     // the ligand's L1 layout depends on the ligand pass: lay it out again
     if (h->ligand_ready) {
       const std::vector<double> pos = h->lig_pos_h, w = h->lig_w_h;
-      return pd_scan_set_ligand(h, (int)(pos.size() / 3), pos.data(), w.data());
+      const int n = (int)(pos.size() / 3);
+      pd_status st = pd_scan_set_ligand(h, n, pos.data(), w.data());
+      if (st != PD_OK) {
+        // keep the handle usable: the previous choice and its layout
+        const std::string msg = pd_detail::last_error();
+        h->tune = old;
+        h->lig_rows = old_rows;
+        pd_scan_set_ligand(h, n, pos.data(), w.data());
+        return fail(st, msg);
+      }
     }
   }
   return PD_OK;
