@@ -111,3 +111,41 @@ def test_ljcomp_two_copies_double_the_energies(oracle):
                 assert p1["offset"][r, i] == p2["offset"][r, i]
                 same += 1
     assert same >= len(rots) * 10 // 2
+
+
+# ljAvg (potentialName "ljAvg", MolForceLJ, docking_molforce.hpp:377-450): one
+# PD_FIELD_LJ_FULL term through pd_ljavg_params, both mixing rules.
+LJAVG_CASES = [
+    ((48, 48, 48), 1.5, 700, 220, 1, 0),      # edge-specialised kernels, geometric sigma
+    ((45, 50, 75), 1.5, 700, 200, 0, 0),      # odd last edge (generic kernels), arithmetic sigma
+    ((64, 64, 64), 1.2, 1800, 450, 1, 1),     # split plane kernels
+]
+
+
+def ljavg_case(oracle, n, h, n_rec, n_lig, mix, seed):
+    cx = synth.make_complex(n_rec, n_lig, seed)
+    rt, w = synth.terms_ljavg(cx, alpha=(0.2, 0.5), cutoff=9.0, mix=mix)
+    p = synth.lj_params(cx, (0.2, 0.5))
+    nr = p.n_rec
+    par, wo = oracle.ljavg_params(p.eps[:nr], p.sigma[:nr], p.alpha[:nr], p.eps[nr:], p.sigma[nr:], mix, 9.0)
+    assert np.array_equal(rt[0][3], par) and np.array_equal(w.reshape(-1), wo.reshape(-1))
+    R = oracle_receptor(oracle, n, h, cx.receptor.pos, rt)
+    return cx, rt, w, R
+
+
+@pytest.mark.parametrize("n,h,n_rec,n_lig,mix,flags", LJAVG_CASES)
+def test_ljavg_receptor_grids_bit_identical(oracle, n, h, n_rec, n_lig, mix, flags):
+    print(f"case ljAvg receptor n={n} mix={mix}", flush=True)
+    cx, rt, w, R = ljavg_case(oracle, n, h, n_rec, n_lig, mix, seed=25)
+    with make_scan(n, h, 1, 4, cx, rt, w, flags=flags) as sc:
+        G = sc.receptor_grids()
+    bad = np.argwhere(G != R)
+    assert bad.shape[0] == 0, f"{bad.shape[0]} cells differ, first {bad[:3]}"
+
+
+@pytest.mark.parametrize("n,h,n_rec,n_lig,mix,flags", LJAVG_CASES)
+def test_ljavg_scan_peaks_match_oracle(oracle, n, h, n_rec, n_lig, mix, flags):
+    print(f"case ljAvg peaks n={n} mix={mix} flags={flags}", flush=True)
+    cx, rt, w, R = ljavg_case(oracle, n, h, n_rec, n_lig, mix, seed=26)
+    checked, ratio = check_peaks_measured_band(oracle, n, h, cx, rt, w, R, flags, n_rot=8, K=10, rot_seed=9)
+    assert checked >= 8 * 10 // 2

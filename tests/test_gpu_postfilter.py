@@ -7,7 +7,8 @@
 (b) TranSymm / TranClust (docking_fft.hpp:30-230) run on the GPU against the
     oracle's restatements applied to the GPU's own candidate list: identical
     output (rotations whose candidates hold exactly equal values, whose
-    TranClust order the reference leaves to std::sort, are counted apart).
+    TranClust order the reference leaves to std::sort, are counted apart), on
+    a whole-plane grid, an odd last edge and a split-plane grid.
 (c) A graph with the post-filters replays identically to the eager scan.
 """
 import numpy as np
@@ -51,6 +52,7 @@ CAND_CASES = [
     ((48, 48, 48), 1.5, 700, 220, 2, 300, 1e16),
     ((40, 36, 45), 1.5, 300, 80, 4, 1000, 1e16),
     ((64, 64, 64), 1.2, 1800, 450, 2, 4096, None),
+    ((32, 256, 256), 1.5, 300, 80, 2, 500, 1e16),  # one-kernel split inverse writes the grid
 ]
 
 
@@ -101,11 +103,15 @@ def _symm_keep(oracle, c, rot, h, pf):
     return keep, rmsd
 
 
+POST_GRIDS = [((48, 48, 48), 700, 220), ((45, 50, 75), 300, 80), ((32, 256, 256), 300, 80)]
+
+
 @pytest.mark.parametrize("mode", ["symm", "clust", "both"])
-def test_postfilters_match_oracle_on_gpu_candidates(oracle, mode):
-    n, h, T, K, k_inp = (48, 48, 48), 1.5, 2, 10, 400
+@pytest.mark.parametrize("n,n_rec,n_lig", POST_GRIDS)
+def test_postfilters_match_oracle_on_gpu_candidates(oracle, mode, n, n_rec, n_lig):
+    h, T, K, k_inp = 1.5, 2, 10, 400
     print(f"case post-filters {mode}: n={n} K={K} k_inp={k_inp}", flush=True)
-    cx, rt, w, R = build_case(oracle, n, h, 700, 220, T, seed=32)
+    cx, rt, w, R = build_case(oracle, n, h, n_rec, n_lig, T, seed=32)
     rng = np.random.default_rng(9)
     rots = _rotations(16, 6)
     pf = dict(n_multimer=2 if mode != "clust" else 0, max_rmsd_symm=8.0, ligand_radius=9.0,
