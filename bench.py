@@ -178,6 +178,8 @@ def main() -> int:
                          "so that a kernel trace times each kernel alone")
     ap.add_argument("--no-pruned", action="store_true",
                     help="skip the proposal-pruned search (second half of the metric)")
+    ap.add_argument("--eager", action="store_true",
+                    help="time eager launches instead of replays of the scan's HIP graph")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--seed", type=int, default=2024)
     args = ap.parse_args()
@@ -228,28 +230,55 @@ def main() -> int:
     d_nout = torch.zeros(n_rot, dtype=torch.int32, device=dev)
     torch.cuda.synchronize()
 
-    def step():
+    def eager_step():
         sc.run_device(n_rot, d_rots.data_ptr(), d_peaks.data_ptr(), d_nout.data_ptr())
+
+    # A step is one scan of the rotation set: by default one replay of the scan's
+    # HIP graph (kernel nodes built once, outside the timed region), else the
+    # same kernels launched eagerly.
+    graph = None
+    if not args.eager:
+        graph = pdfft.Graph([(sc, n_rot, d_rots.data_ptr(), d_peaks.data_ptr(), d_nout.data_ptr(), 0)], 1)
+        step, wait = graph.launch, graph.sync
+    else:
+        step, wait = eager_step, sc.sync
 
     for i in range(args.warmup):
         step()
-        sc.sync()
+        wait()
         log(f"warmup step {i} done")
     torch.cuda.synchronize()
-    sc.sync()
+    wait()
     t0 = time.perf_counter()
     for i in range(args.steps):
         step()
-    sc.sync()
+    wait()
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     ms_per_step = dt / args.steps * 1e3
     rate = n_rot * args.steps / dt
-    log(f"timed {args.steps} steps: {ms_per_step:.1f} ms/step, {rate:.0f} rotations/s")
+    log(f"timed {args.steps} {'graph replays' if graph else 'eager steps'}: {ms_per_step:.1f} ms/step, "
+        f"{rate:.0f} rotations/s")
+    launch = {"mode": "graph" if graph else "eager"}
+    if graph is not None:
+        # the same scan launched eagerly, for comparison (outside the timed steps)
+        graph_peaks = d_peaks.cpu().numpy().copy()
+        graph.close()
+        eager_step()
+        sc.sync()
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            eager_step()
+        sc.sync()
+        dte = time.perf_counter() - t0
+        launch["eager_rotations_per_s"] = n_rot * args.steps / dte
+        launch["graph_identical_to_eager"] = bool(np.array_equal(d_peaks.cpu().numpy(), graph_peaks))
+        log(f"eager, same scan: {launch['eager_rotations_per_s']:.0f} rotations/s; "
+            f"identical: {launch['graph_identical_to_eager']}")
 
     # Per-kernel device time with HIP events on the scan's stream (separate pass).
     sc.set_timing(True)
-    step()
+    eager_step()
     sc.sync()
     stats = sc.kernel_stats()
     sc.set_timing(False)
@@ -318,8 +347,9 @@ def main() -> int:
                         f"{n[0]}x{n[1]}x{n[2]} grid at {h} A, {T} terms, {n_rot} rotations, top-{K} "
                         f"per rotation, batch {args.batch}",
             "grid": list(n), "terms": T, "rotations": n_rot, "top_k": K, "batch": args.batch,
-            "seed": args.seed,
+            "seed": args.seed, "launch": launch["mode"],
         },
+        "launch": launch,
         "roofline": {
             "bound": "hbm",
             "kernel": dom,
