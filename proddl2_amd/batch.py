@@ -125,15 +125,16 @@ def _tuning_key(device, edge):
     return f"{device}|{edge}^3|T{N_TERMS}|K{K}"
 
 
-def load_tuning(path=TUNING_FILE):
+def load_tuning(path=None):
     try:
-        with open(path) as f:
+        with open(path or TUNING_FILE) as f:
             return json.load(f)
     except (OSError, ValueError):
         return {}
 
 
-def save_tuning(table, path=TUNING_FILE):
+def save_tuning(table, path=None):
+    path = path or TUNING_FILE
     os.makedirs(os.path.dirname(path), exist_ok=True)
     tmp = path + ".tmp"
     with open(tmp, "w") as f:

@@ -238,3 +238,35 @@ def test_built_graph_uneven_pairs(dev, oracle):
         assert checked >= batch.K
     for p in pairs:
         p.close()
+
+
+def test_autotune_keeps_bits_and_persists(dev, tmp_path, monkeypatch):
+    """batch.autotune on three uneven pairs: every tuned pair's peaks equal its
+    untuned peaks byte for byte, the choices land in the tuning file, and a
+    second call takes them from there without timing."""
+    from proddl2_amd import batch
+    path = str(tmp_path / "tuning.json")
+    monkeypatch.setattr(batch, "TUNING_FILE", path)
+    specs = batch.uneven_pairs(3, 192, 3100)
+    pairs = [batch.Pair(s, torch, dev) for s in specs]
+    print(f"case autotune: edges {[s['edge'] for s in specs]}", flush=True)
+    untuned = []
+    for p in pairs:
+        p.run()
+        untuned.append(p.peaks())
+    logs = []
+    tuned, source, _ = batch.autotune(pairs, torch, lanes=(), sample=128, log=logs.append)
+    assert set(source.values()) == {"tuned"}
+    for p, (pk, no) in zip(pairs, untuned):
+        p.d_peaks.zero_()
+        p.run()
+        gp, gn = p.peaks()
+        assert np.array_equal(gn, no) and np.array_equal(gp.view(np.uint8), pk.view(np.uint8)), p.spec
+    kept = batch.load_tuning(path)
+    assert len(kept) == len(tuned)
+    tuned2, source2, _ = batch.autotune(pairs, torch, lanes=(), sample=128, log=logs.append)
+    assert set(source2.values()) == {"kept"}
+    assert {e: {k: v for k, v in r.items() if k != "effective"} for e, r in tuned2.items()} == \
+        {e: {k: v for k, v in r.items() if k != "effective"} for e, r in tuned.items()}
+    for p in pairs:
+        p.close()
