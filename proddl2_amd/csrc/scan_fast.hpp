@@ -505,8 +505,19 @@ __global__ __launch_bounds__(256, PD_COL_WAVES) void k_col_fast(ScanGeom g, cons
 
 // ---------------------------------------------------------------------------
 // Ligand plane pass.  grid (p0, nrot), block 512, LDS N1 * LS complex.
-constexpr int kLigList = 1024;  // atoms of one plane kept in the block's list
-constexpr int kLigTerms = 4;    // terms whose weights the list keeps in LDS
+constexpr int kLigTerms = 4;  // terms whose weights the plane's atom list keeps in LDS
+// Atoms of one plane the list holds: as many as keep the block within 80 KB
+// of LDS (two blocks per CU: 128^3 ligand pass 198 -> 156 us per launch of 64
+// rotations), at least 384, at most 1024 (where the plane alone takes more
+// than 80 KB).  A plane with more atoms is summed from the atom cells.
+template <int N1, int N2>
+struct LigList {
+  static constexpr int kBuf = N1 * ((N2 / 2 + 1) | 1) * 8;        // dynamic: the plane
+  static constexpr int kFixed = N2 * 8 + (2 * N1 + 1) * 4 + 256;   // twiddles, row starts, cursors
+  static constexpr int kPer = 8 + 4 * kLigTerms;                   // bytes per listed atom
+  static constexpr int kFit = (80 * 1024 - kBuf - kFixed) / kPer / 32 * 32;
+  static constexpr int value = kFit >= 384 ? (kFit < 1024 ? kFit : 1024) : 1024;
+};
 // ---------------------------------------------------------------------------
 template <int N1, int N2>
 __global__ __launch_bounds__(512, PD_LIG_WAVES) void k_lig_plane_fwd_fast(ScanGeom g, const float2* __restrict__ twN1,
@@ -517,13 +528,9 @@ __global__ __launch_bounds__(512, PD_LIG_WAVES) void k_lig_plane_fwd_fast(ScanGe
                                                             float2* __restrict__ L1) {
   constexpr int M2 = N2 / 2, H2 = M2 + 1, LS = H2 | 1, NT = 512;
   using SY = Split<N1>;
-  using SZ = Split<M2>;
   constexpr int A1 = SY::A, B1 = SY::B, J1 = SY::J;
-  constexpr int A2 = SZ::A, B2 = SZ::B, J2 = SZ::J;
   constexpr int LY = NT / B1, RY = (M2 + LY - 1) / LY;  // y columns: M2 (0 and M2 packed)
-  constexpr int LZ = NT / B2, RZ = (N1 + LZ - 1) / LZ;
   extern __shared__ float2 buf[];
-  float* fb = reinterpret_cast<float*>(buf);
   const int tid = threadIdx.x;
   // Blocks go to the 8 XCDs round-robin in launch order.  The 16 planes of one
   // interleaved group share each 512-byte block of L1 in 32-byte pieces, so
@@ -539,135 +546,135 @@ __global__ __launch_bounds__(512, PD_LIG_WAVES) void k_lig_plane_fwd_fast(ScanGe
   const int lo1 = g.lo[1], p1 = g.p[1];
   const int plane = N1 * N2;
   const int* cb = cells + (size_t)b * n_atoms;
-  const int gz = tid / LZ, rz = tid % LZ, gy = tid / LY, cy = tid % LY;
-  // LZ is a multiple of 64, so gz is wave-uniform: as a uniform value it makes
-  // the z pass's twiddle reads scalar loads, off the vector memory path.  (In
-  // the y pass, whose loads come from LDS, scalar twiddle reads measured
-  // slower: they share the LDS wait counter.)
-  static_assert(LZ % 64 == 0, "one lane group per wave");
-  const int gzu = __builtin_amdgcn_readfirstlane(gz);
-  // The plane's atoms, found once for all terms: atom index, position in the
-  // plane buffer (as a float index) and, for up to kLigTerms terms, the
-  // weights, so that the deposits of later terms read LDS only.  More than
-  // kLigList atoms in one plane fall back to scanning every atom per term.
-  __shared__ int s_cnt;
-  __shared__ int s_atom[kLigList];
-  __shared__ int s_pos[kLigList];
+  const int gy = tid / LY, cy = tid % LY;
+  const int lane = tid & 63, wave = tid >> 6;
+  // The z spectra of the plane's box rows by direct summation over each row's
+  // atoms (a handful per row, against a length-N2 transform of every row):
+  //   X_t[row][kz] = sum_j w_tj exp(-2 pi i kz z_j / N2),
+  // summed in ascending atom order, so that every run rounds alike.  The
+  // plane's atoms are listed by box row (counted, placed with atomics, then
+  // each row put in atom order), with the weights of up to kLigTerms terms.
+  // A plane with more than kLigList atoms is summed from the atom cells
+  // directly (same order, slower).
+  __shared__ float2 s_tw[N2];
+  __shared__ int s_row[N1 + 1];  // row r's atoms: [s_row[r], s_row[r + 1])
+  __shared__ int s_cur[N1];
+  constexpr int kLigList = LigList<N1, N2>::value;
+  __shared__ int s_z[kLigList];
+  __shared__ int s_ra[kLigList];
   __shared__ float s_w[kLigTerms][kLigList];
   const bool wl = g.T <= kLigTerms;
-  if (tid == 0) s_cnt = 0;
+  for (int i = tid; i < N2; i += NT) s_tw[i] = tw2[i];
+  for (int i = tid; i <= p1; i += NT) s_row[i] = 0;
   __syncthreads();
+  // box row * N2 + z of atom j if it lies in this plane, else -1
+  auto key_of = [&](int j) -> int {
+    const int off = cb[j];
+    if (off < 0 || off / plane != i0) return -1;
+    const int r = off - i0 * plane;
+    const int y = r / N2;
+    return (y - lo1) * N2 + (r - y * N2);
+  };
 #pragma unroll 4
   for (int j = tid; j < n_atoms; j += NT) {
-    const int off = cb[j];
-    if (off >= 0 && off / plane == i0) {
-      const int r = off - i0 * plane;
-      const int i1 = r / N2, i2 = r - i1 * N2;
-      const int k = atomicAdd(&s_cnt, 1);
-      if (k < kLigList) {
-        s_atom[k] = j;
-        s_pos[k] = i1 * 2 * LS + i2;
-        if (wl)
-          for (int t = 0; t < g.T; ++t) s_w[t][k] = w[(size_t)t * n_atoms + j];
-      }
-    }
+    const int k = key_of(j);
+    if (k >= 0) atomicAdd(&s_row[k / N2 + 1], 1);
   }
   __syncthreads();
-  const int cnt = s_cnt;
+  if (tid == 0)
+    for (int i = 1; i <= p1; ++i) s_row[i] += s_row[i - 1];
+  __syncthreads();
+  const bool listed = s_row[p1] <= kLigList;
+  if (listed) {
+    for (int i = tid; i < p1; i += NT) s_cur[i] = s_row[i];
+    __syncthreads();
+#pragma unroll 4
+    for (int j = tid; j < n_atoms; j += NT) {
+      const int k = key_of(j);
+      if (k >= 0) {
+        const int r = k / N2;
+        const int pos = atomicAdd(&s_cur[r], 1);
+        s_z[pos] = k - r * N2;
+        s_ra[pos] = j;
+        if (wl)
+          for (int t = 0; t < g.T; ++t) s_w[t][pos] = w[(size_t)t * n_atoms + j];
+      }
+    }
+    __syncthreads();
+    // each row in atom order (insertion sort of its few entries)
+    for (int r = tid; r < p1; r += NT) {
+      const int a0 = s_row[r], a1 = s_row[r + 1];
+      for (int i = a0 + 1; i < a1; ++i) {
+        const int ja = s_ra[i], za = s_z[i];
+        float wa[kLigTerms];
+        for (int t = 0; t < kLigTerms; ++t) wa[t] = wl && t < g.T ? s_w[t][i] : 0.f;
+        int q = i - 1;
+        while (q >= a0 && s_ra[q] > ja) {
+          s_ra[q + 1] = s_ra[q];
+          s_z[q + 1] = s_z[q];
+          if (wl)
+            for (int t = 0; t < g.T; ++t) s_w[t][q + 1] = s_w[t][q];
+          --q;
+        }
+        s_ra[q + 1] = ja;
+        s_z[q + 1] = za;
+        if (wl)
+          for (int t = 0; t < g.T; ++t) s_w[t][q + 1] = wa[t];
+      }
+    }
+    __syncthreads();
+  }
+  constexpr int NQ = (M2 + 63) / 64;  // columns per lane in the summation
   for (int t = 0; t < g.T; ++t) {
-    for (int e = tid; e < p1 * M2; e += NT) buf[(lo1 + e / M2) * LS + e % M2] = make_float2(0.f, 0.f);
-    __syncthreads();
+    // Box rows of term t into buf, one row per wave at a time, lane c taking
+    // columns c + 64 q: slot c in [1, M2) holds X[c], slot 0 holds
+    // X[0] + i X[M2] (both real), the packed column the y pass takes.
     const float* wt = w + (size_t)t * n_atoms;
-    if (cnt <= kLigList && wl) {
-      for (int k = tid; k < cnt; k += NT) atomicAdd(&fb[s_pos[k]], s_w[t][k]);
-    } else if (cnt <= kLigList) {
-      for (int k = tid; k < cnt; k += NT) atomicAdd(&fb[s_pos[k]], wt[s_atom[k]]);
-    } else {
-      for (int j = tid; j < n_atoms; j += NT) {
-        const int off = cb[j];
-        if (off >= 0 && off / plane == i0) {
-          const int r = off - i0 * plane;
-          const int i1 = r / N2, i2 = r - i1 * N2;
-          atomicAdd(&fb[i1 * 2 * LS + i2], wt[j]);
+    for (int r = wave; r < p1; r += NT / 64) {
+      float2 a[NQ];
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) a[q] = make_float2(0.f, 0.f);
+      auto add = [&](int z, float wv) {
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+          const int c = lane + 64 * q;
+          const float2 ph = s_tw[(c * z) % N2];
+          const float py = c == 0 ? s_tw[(M2 * z) % N2].x : ph.y;
+          a[q].x = fmaf(wv, ph.x, a[q].x);
+          a[q].y = fmaf(wv, py, a[q].y);
         }
+      };
+      if (listed) {
+        const int a0 = s_row[r], a1 = s_row[r + 1];
+        for (int i = a0; i < a1; ++i) add(s_z[i], wl ? s_w[t][i] : wt[s_ra[i]]);
+      } else {
+        for (int j = 0; j < n_atoms; ++j) {
+          const int k = key_of(j);
+          if (k >= 0 && k / N2 == r) add(k - r * N2, wt[j]);
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        const int c = lane + 64 * q;
+        if (c < M2) buf[(lo1 + r) * LS + c] = a[q];
       }
     }
     __syncthreads();
-    // z pass: packed half-length forward FFT of rows lo1 .. lo1+p1-1
-    {
-      float2 z[RZ][A2];
-#pragma unroll
-      for (int r = 0; r < RZ; ++r) {
-        const int rr = r * LZ + rz;
-        if (rr < p1) {
-          const int row = lo1 + rr;
-#pragma unroll
-          for (int m = 0; m < A2; ++m) z[r][m] = buf[row * LS + gz + B2 * m];
-        }
-      }
-      __syncthreads();
-#pragma unroll
-      for (int r = 0; r < RZ; ++r) {
-        const int rr = r * LZ + rz;
-        if (rr < p1) {
-          const int row = lo1 + rr;
-          four_step_phase1<A2, B2, -1>(z[r], gzu, twM2);
-#pragma unroll
-          for (int k1 = 0; k1 < A2; ++k1) buf[row * LS + k1 * B2 + gz] = z[r][k1];
-        }
-      }
-      __syncthreads();
-      float2 u[RZ][J2][B2];
-#pragma unroll
-      for (int r = 0; r < RZ; ++r) {
-        const int rr = r * LZ + rz;
-#pragma unroll
-        for (int j = 0; j < J2; ++j) {
-          const int k1 = gz + B2 * j;
-          if (rr < p1 && k1 < A2)
-#pragma unroll
-            for (int q = 0; q < B2; ++q) u[r][j][q] = buf[(lo1 + rr) * LS + k1 * B2 + q];
-        }
-      }
-      __syncthreads();
-#pragma unroll
-      for (int r = 0; r < RZ; ++r) {
-        const int rr = r * LZ + rz;
-#pragma unroll
-        for (int j = 0; j < J2; ++j) {
-          const int k1 = gz + B2 * j;
-          if (rr < p1 && k1 < A2) {
-            rdft<B2, -1>(u[r][j]);
-#pragma unroll
-            for (int k2 = 0; k2 < B2; ++k2) buf[(lo1 + rr) * LS + k1 + A2 * k2] = u[r][j][k2];
-          }
-        }
-      }
-      __syncthreads();
-    }
-    // y pass: r2c post-step on the fly, forward FFT along y, store to HBM.
-    // Columns 0 and M2 are real along y; they travel packed as one complex
-    // column c = X[.][0] + i X[.][M2] in column slot 0, so the pass covers M2
-    // columns.  Their spectra are split afterwards by Hermitian symmetry.
+    // y pass: forward FFT along y, store to HBM.  Columns 0 and M2 are real
+    // along y; they travel packed as one complex column c = X[.][0] + i X[.][M2]
+    // in column slot 0, so the pass covers M2 columns.  Their spectra are split
+    // afterwards by Hermitian symmetry.
     {
       float2 v[RY][A1];
 #pragma unroll
       for (int r = 0; r < RY; ++r) {
         const int col = r * LY + cy;
         if (col < M2) {
-          const float2 wz = tw2[col];
-          const int cm = (M2 - col) % M2;
 #pragma unroll
           for (int m = 0; m < A1; ++m) {
             const int i1 = gy + B1 * m;
             const int rr = i1 - lo1;
-            float2 x = make_float2(0.f, 0.f);
-            if (rr >= 0 && rr < p1) {
-              const float2 zk = buf[i1 * LS + col];
-              // column 0: X[0] = Re Z0 + Im Z0, X[M2] = Re Z0 - Im Z0 (both real)
-              x = col == 0 ? make_float2(zk.x + zk.y, zk.x - zk.y) : r2c_point(zk, buf[i1 * LS + cm], wz);
-            }
-            v[r][m] = x;
+            v[r][m] = rr >= 0 && rr < p1 ? buf[i1 * LS + col] : make_float2(0.f, 0.f);
           }
         }
       }
