@@ -312,9 +312,9 @@ struct pd_scan {
     hipStream_t s = nullptr;
     hipEvent_t done = nullptr;
   };
-  // Three lanes: measured +1% at 128^3 and +3% at 256^3 over two; four were
-  // slower (with the handle's stream, three lanes fill the four hardware
-  // queues a process gets by default).
+  // Up to three lanes (default_lanes picks two or three); four were slower
+  // (with the handle's stream, three lanes fill the four hardware queues a
+  // process gets by default).
   static constexpr int kLanes = 3;
   Lane lane[kLanes];
   hipEvent_t fork = nullptr;
@@ -539,17 +539,18 @@ ColShape col_shape(const pd_scan* h, int nb) {
   const int ntile = (g.M + C - 1) / C;
   int nyb = std::max(1, std::min(nb, 4096 / std::max(1, ntile)));
   if (fc) {
-    // About 1,024 blocks: the rotation groups of a tile run together on one
-    // XCD and share each read of the tile's receptor spectra through L2,
-    // and fewer, longer blocks leave CUs to the other lanes' plane passes.
-    // Measured with three lanes at 128^3 (260 tiles): 4 groups of 16
-    // rotations 91.4-91.7k rotations/s, 8 groups 90.2-91.5k, 16 groups
-    // 89.7-90.4k, 2 groups 89.6-90.2k; the column pass timed alone is
-    // fastest with 16 groups (158 against 190 ms per 30,000 rotations).
-    // 256^3 (2,064 tiles) gets 2 groups of 32, the fewest its 32-bit
-    // offsets allow (column pass 525 ms per 10,000 rotations against 607
-    // with 16 groups of 4).
-    nyb = std::max(1, std::min((nb + 1) / 2, (1024 + ntile / 2) / ntile));
+    // About 4,096 blocks: the rotation groups of a tile run together on one
+    // XCD and share each read of the tile's receptor spectra through L2.
+    // Measured with the direct-summation ligand pass (two blocks per CU) and
+    // the default lanes: 128^3 (260 tiles) 16 groups of 4 rotations 96.7k
+    // rotations/s against 89.4-96.0k with 4 groups of 16 (the lower figure
+    // with three lanes where two of them shared a hardware queue); 96^3
+    // (147 tiles) 16 groups 158k against 147k with the earlier 1,024-block
+    // rule; 160^3 (810 tiles) 5 groups 21.4k against 20.8k with 1.  256^3
+    // (2,064 tiles) gets 2 groups of 32, the fewest its 32-bit offsets allow
+    // (column pass 525 ms per 10,000 rotations against 607 with 16 groups
+    // of 4).
+    nyb = std::max(1, std::min((nb + 1) / 2, (4096 + ntile / 2) / ntile));
   }
   if (h->tune.col_groups > 0) nyb = std::min(nb, h->tune.col_groups);
   if (fc) nyb = std::max(nyb, (int)((nb + max_bpb - 1) / max_bpb));
@@ -689,12 +690,21 @@ pd_status check_run(pd_scan* h, int n_rot) {
   return PD_OK;
 }
 
+// Batch lanes by default: two up to N0 = 128, three above.  The lanes and the
+// handle's stream take three of the four hardware queues a process gets, so a
+// caller's own stream (torch's, for one) never shares a queue with a lane;
+// with three lanes at 128^3 that sharing took the scan from 96.0k to 89.4k
+// rotations/s, two lanes ran 96.7k either way (96^3: 158k against 152k).
+// Grids above 128 keep three (160^3: 21.6k against 21.4k; 256^3: 9.9k
+// against 9.7k).
+int default_lanes(const pd_scan* h) { return h->g.n0 > 128 ? pd_scan::kLanes : 2; }
+
 // Lanes a scan of nbatch batches uses: one while kernels are timed (so that
 // each recorded duration belongs to one kernel alone), with PD_SCAN_ONE_LANE,
 // or for a single batch.
 int lanes_for(const pd_scan* h, int nbatch) {
   const bool single = h->timing || nbatch <= 1 || (h->d.flags & PD_SCAN_ONE_LANE) != 0;
-  return single ? 1 : (h->tune.lanes > 0 ? h->tune.lanes : pd_scan::kLanes);
+  return single ? 1 : (h->tune.lanes > 0 ? h->tune.lanes : default_lanes(h));
 }
 
 // Eager scan on the handle's stream: batches go round the lanes' streams,

@@ -37,9 +37,10 @@ def test_bitwise_choices_keep_results(oracle, n, h, n_rec, n_lig):
         ref = _run(sc, rots)
         base = sc.tuning()
         print(f"  rules: {base}", flush=True)
-        assert base["col_groups"] >= 1 and base["lanes"] == 3 and base["col_mz_extra"] == 0
+        # lanes: two by default up to N0 = 128, three above
+        assert base["col_groups"] >= 1 and base["lanes"] == (3 if n[0] > 128 else 2) and base["col_mz_extra"] == 0
         for kw in ({"col_groups": 1}, {"col_groups": 2}, {"col_groups": 16}, {"col_groups": 64},
-                   {"lanes": 1}, {"lanes": 2}, {"col_groups": 8, "lanes": 2}):
+                   {"lanes": 1}, {"lanes": 2}, {"lanes": 3}, {"col_groups": 8, "lanes": 2}):
             sc.set_tuning(**kw)
             eff = sc.tuning()
             if "lanes" in kw:
