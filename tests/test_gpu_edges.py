@@ -138,3 +138,39 @@ def test_zero_weights_tie_everywhere(oracle):
         sc.set_ligand(cx.ligand.pos, w0)
         peaks, nout = sc.run(rots)
         assert (nout == 0).all()
+
+
+def test_dense_plane_beyond_the_atom_list(oracle):
+    """A flat ligand of 900 atoms in one x plane (0.9 A apart in y and z, so
+    some 1.2 A cells hold two atoms): more than the 512 atoms the 128^3 ligand
+    pass lists per plane, so that plane's z spectra are summed from the atom
+    cells directly.  Rotations about x keep the disk in its plane.  Grids
+    within the bar of the oracle's, and the same bits at batch 1 as at batch 4."""
+    from tests.parity_helpers import check_grids, oracle_receptor
+    n, h, T = (128, 128, 128), 1.2, 4
+    print(f"case dense ligand plane n={n}", flush=True)
+    cx = synth.make_complex(1500, 60, 71)
+    rt, _ = synth.terms_for(cx, T)
+    g = np.arange(-15, 16) * 0.9
+    yy, zz = np.meshgrid(g, g, indexing="ij")
+    keep = yy ** 2 + zz ** 2 <= 13.6 ** 2
+    disk = np.stack([np.full(keep.sum(), 0.3), yy[keep], zz[keep]], axis=1)
+    assert disk.shape[0] > 512
+    w = np.random.default_rng(72).uniform(-1.0, 1.0, (T, disk.shape[0]))
+    R = oracle_receptor(oracle, n, h, cx.receptor.pos, rt)
+    S = oracle.receptor_spectra(R)
+    rots = []
+    for a in (0.0, 0.7, 2.1):
+        c, s = np.cos(a), np.sin(a)
+        rots.append(np.array([[1, 0, 0], [0, c, -s], [0, s, c]], dtype=np.float64))
+    rots = np.stack(rots)
+    sc = pdfft.Scan(n, h, T, max_n_trans=10, max_val_corr=1e16, batch=4)
+    with sc:
+        sc.project_receptor(cx.receptor.pos, rt)
+        sc.set_ligand(disk, w)
+        check_grids(oracle, sc, n, h, disk, w, S, R, rots)
+        ref = sc.run(rots)
+        sc.set_batch(1)
+        got = sc.run(rots)
+    assert np.array_equal(ref[0].view(np.uint8), got[0].view(np.uint8)) and np.array_equal(ref[1], got[1])
+    print(f"  {disk.shape[0]} atoms in one plane: grids within the bar, batch 1 == batch 4", flush=True)
