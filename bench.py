@@ -299,6 +299,12 @@ def main() -> int:
     _, _, pp = sc.ligand_box()
     pruned_bytes = T * S * pp / n[0] + T * S / args.batch + 2 * S
     summ = counter_summary(args.workload, args.batch) or {}
+    pipe_traffic = None
+    tb = summ.get("traffic_bytes_per_launch") or {}
+    if tb and all(k in stats and stats[k][1] for k in tb):
+        per_rot_bytes = sum(v * stats[k][1] for k, v in tb.items()) / n_rot
+        pipe_traffic = {"bytes_per_rotation": per_rot_bytes, "achieved_GBs": rate * per_rot_bytes / 1e9,
+                        "frac": rate * per_rot_bytes / 1e9 / HBM_PEAK_GBS}
     executed = {}
     for kname, fl in (summ.get("fp32_flops_per_launch") or {}).items():
         if kname in stats and stats[kname][1]:
@@ -323,6 +329,8 @@ def main() -> int:
         pruned = pruned_search(sc, cx, d_rots, peaks, ms_per_step / 1e3, log)
 
     hbm_copy = copy_bandwidth(torch, dev)
+    if pipe_traffic:
+        pipe_traffic["frac_of_copy"] = pipe_traffic["achieved_GBs"] / hbm_copy["GBs"]
     log(f"device copy: {hbm_copy['GBs']:.0f} GB/s (read + write)")
     threads = args.cpu_threads or min(16, os.cpu_count() or 1)
     cpu = None
@@ -364,7 +372,13 @@ def main() -> int:
             "traffic_GBs": traffic / avg_s / 1e9 if traffic else None,
             "traffic_frac": traffic / avg_s / 1e9 / HBM_PEAK_GBS if traffic else None,
         },
-        "pipeline_roofline": {
+        # the whole scan priced by the bytes the counters saw moving (every scan
+        # kernel's calibrated HBM bytes per launch, per rotation) at the step rate
+        "pipeline_traffic": pipe_traffic,
+        # the section 8(d) model (the ligand's T spectra over the whole grid)
+        # counts planes the ligand box never fills: above 1 once the scan
+        # outruns the model's ceiling, so it is not a bandwidth figure
+        "pipeline_model_bytes": {
             "bytes_per_rotation": total_bytes,
             "achieved_GBs": rate * total_bytes / 1e9,
             "frac": rate * total_bytes / 1e9 / HBM_PEAK_GBS,

@@ -701,6 +701,13 @@ __global__ __launch_bounds__(512, PD_LIG_WAVES) void k_lig_plane_fwd_fast(ScanGe
       auto off8 = [&](int m) -> uint32_t {  // byte offset of column m of this plane
         return g.il ? (qb + ((uint32_t)(m >> g.ilsh) << 6) + ((uint32_t)m & cmask)) * 8u : (qb + (uint32_t)m) * 8u;
       };
+      // Column m + k2 A1 H2 lies k2 kstep bytes past column m (the same for
+      // every lane) when A1 H2 is a whole number of the interleave's column
+      // blocks (the interleave is the column pass's B = ColSplit<N1>::B lanes
+      // on these cubic grids, or none): then one offset per lane and a scalar
+      // offset per store replace the per-store address arithmetic.
+      constexpr bool lin = (A1 * H2) % (64 / ColSplit<N1>::B) == 0;
+      const uint32_t kstep = g.il ? ((uint32_t)(A1 * H2) >> g.ilsh) * 512u : (uint32_t)(A1 * H2) * 8u;
 #pragma unroll
       for (int r = 0; r < RY; ++r) {
         const int col = r * LY + cy;
@@ -713,8 +720,14 @@ __global__ __launch_bounds__(512, PD_LIG_WAVES) void k_lig_plane_fwd_fast(ScanGe
             for (int q = 0; q < B1; ++q) x[q] = buf[(k1 * B1 + q) * LS + col];
             rdft<B1, -1>(x);
             if (col != 0) {
+              if constexpr (lin) {
+                const uint32_t o = off8(k1 * H2 + col);
 #pragma unroll
-              for (int k2 = 0; k2 < B1; ++k2) pd_st2(rd, off8((k1 + A1 * k2) * H2 + col), 0u, x[k2]);
+                for (int k2 = 0; k2 < B1; ++k2) pd_st2(rd, o, (uint32_t)k2 * kstep, x[k2]);
+              } else {
+#pragma unroll
+                for (int k2 = 0; k2 < B1; ++k2) pd_st2(rd, off8((k1 + A1 * k2) * H2 + col), 0u, x[k2]);
+              }
             } else {
               // packed spectrum C[k] parked in the unused slot column M2
 #pragma unroll
