@@ -336,7 +336,9 @@ __global__ __launch_bounds__(256, PD_COL_WAVES) void k_col_fast(ScanGeom g, cons
   float2* twl = xbuf + 4 * WX;
   for (int i = threadIdx.x; i < N0; i += blockDim.x) twl[i] = twN0[i];
   __syncthreads();
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  // wave index as a uniform value: the receptor loads' scalar offsets depend
+  // on it, and taken from threadIdx each would need a waterfall loop
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int lg = lane / CW, cl = lane % CW;
   __builtin_assume(lg >= 0 && lg < B);
   float2* xb = xbuf + wave * WX;
