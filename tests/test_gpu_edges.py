@@ -174,3 +174,22 @@ def test_dense_plane_beyond_the_atom_list(oracle):
         got = sc.run(rots)
     assert np.array_equal(ref[0].view(np.uint8), got[0].view(np.uint8)) and np.array_equal(ref[1], got[1])
     print(f"  {disk.shape[0]} atoms in one plane: grids within the bar, batch 1 == batch 4", flush=True)
+
+
+def test_six_terms_weights_beyond_the_list(oracle):
+    """Six terms (the four-term set and the two-term set together): more terms
+    than the ligand pass keeps weights of in its atom list, so it reads them
+    from the weight array.  Grids within the bar and peaks against the oracle."""
+    from tests.parity_helpers import check_grids, check_peaks_terms, oracle_receptor
+    n, h = (48, 48, 48), 1.2
+    print(f"case six terms n={n}", flush=True)
+    cx = synth.make_complex(700, 220, 73)
+    rt4, w4 = synth.terms_four(cx)
+    rt2, w2 = synth.terms_shape_elec(cx)
+    rt, w = rt4 + rt2, np.concatenate([w4, w2])
+    R = oracle_receptor(oracle, n, h, cx.receptor.pos, rt)
+    S = oracle.receptor_spectra(R)
+    with make_scan(n, h, 6, 10, cx, rt, w) as sc:
+        check_grids(oracle, sc, n, h, cx.ligand.pos, w, S, R, synth.rotation_set(2, seed=74))
+    checked = check_peaks_terms(oracle, n, h, cx, rt, w, R, 0, n_rot=6)
+    print(f"  {checked} ranks exact outside the band", flush=True)
