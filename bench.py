@@ -375,7 +375,8 @@ def main() -> int:
         # the whole scan priced by the bytes the counters saw moving (every scan
         # kernel's calibrated HBM bytes per launch, per rotation) at the step rate
         "pipeline_traffic": pipe_traffic,
-        # the section 8(d) model (the ligand's T spectra over the whole grid)
+        # the algorithmic model T*S + T*S/B + S per rotation (the ligand's T
+        # spectra over the whole grid)
         # counts planes the ligand box never fills: above 1 once the scan
         # outruns the model's ceiling, so it is not a bandwidth figure
         "pipeline_model_bytes": {

@@ -5,7 +5,7 @@
  * rotational grid (docking_app.hpp:487-513).  This network ranks the
  * rotations of one complex before the scan so that only a top fraction gets
  * the full FFT correlation (pd_scan_run_device on the kept rotations).  The
- * exhaustive scan stays available as fallback and as the judge of recall.
+ * exhaustive scan stays available as fallback and as the measure of recall.
  *
  * Model (fixed shapes):
  *  - per molecule, two 16^3 interatomic-vector (Patterson) maps on a grid of

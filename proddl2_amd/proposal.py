@@ -6,7 +6,7 @@ The reference worker correlates every rotation of its grid
 the best exhaustive energy of each rotation of many synthetic pairs, ranks the
 rotations of a new complex; only the top fraction is correlated
 (pd_scan_run_device on the gathered rotations), and the exhaustive scan is the
-fallback and the judge (recall of its top poses).
+fallback and the measure of recall (the share of its top poses kept).
 
 Everything runs through libpdfft.so; there is no CPU path.
 
