@@ -375,8 +375,8 @@ def main() -> int:
         # the whole scan priced by the bytes the counters saw moving (every scan
         # kernel's calibrated HBM bytes per launch, per rotation) at the step rate
         "pipeline_traffic": pipe_traffic,
-        # the algorithmic model T*S + T*S/B + S per rotation (the ligand's T
-        # spectra over the whole grid)
+        # the algorithmic model (2T + 2) S + T S / B per rotation (the ligand's T
+        # spectra over the whole grid, written and read; S written and read)
         # counts planes the ligand box never fills: above 1 once the scan
         # outruns the model's ceiling, so it is not a bandwidth figure
         "pipeline_model_bytes": {
