@@ -163,6 +163,19 @@ pd_status fail(pd_status s, const std::string& msg) {
       return fail(PD_ERR_DEVICE, std::string(#x) + " failed: " + hipGetErrorString(e_)); \
   } while (0)
 
+// Device memory of one call, freed on every return path (hipFree waits for the
+// work queued on it).
+template <typename T>
+struct CallBuf {
+  T* p = nullptr;
+  CallBuf() = default;
+  CallBuf(const CallBuf&) = delete;
+  CallBuf& operator=(const CallBuf&) = delete;
+  ~CallBuf() {
+    if (p) hipFree(p);
+  }
+};
+
 constexpr int kPlaneThreads = 512;
 constexpr int kColThreads = 256;
 constexpr int kMaxLds = 160 * 1024;
@@ -476,10 +489,12 @@ int col_lds(const pd_scan* h) { return h->g.n0 * h->C * (int)sizeof(float2); }
 // Receptor forward transform of the FP64 grids already in h->R64.
 pd_status receptor_spectra(pd_scan* h) {
   const ScanGeom& g = h->g;
-  float* r32 = nullptr;
-  float2* tmp = nullptr;
-  PD_HIP(hipMalloc(&r32, sizeof(float) * h->N * g.T));
-  PD_HIP(hipMalloc(&tmp, sizeof(float2) * (size_t)g.n0 * g.M));
+  CallBuf<float> r32b;
+  CallBuf<float2> tmpb;
+  PD_HIP(hipMalloc(&r32b.p, sizeof(float) * h->N * g.T));
+  PD_HIP(hipMalloc(&tmpb.p, sizeof(float2) * (size_t)g.n0 * g.M));
+  float* const r32 = r32b.p;
+  float2* const tmp = tmpb.p;
   const long long tot = h->N * g.T;
   hipLaunchKernelGGL(k_to_float, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, h->stream, h->R64,
                      r32, tot);
@@ -512,8 +527,6 @@ pd_status receptor_spectra(pd_scan* h) {
     PD_HIP(hipGetLastError());
   }
   PD_HIP(hipStreamSynchronize(h->stream));
-  hipFree(r32);
-  hipFree(tmp);
   h->rhat_shift = g.pl;
   h->receptor_ready = true;
   return PD_OK;
@@ -722,20 +735,21 @@ pd_status run_batches(pd_scan* h, int n_rot, const double* d_rots, int rot0, Pea
     PD_HIP(hipEventRecord(h->fork, h->stream));
     for (int l = 0; l < nl; ++l) PD_HIP(hipStreamWaitEvent(h->lane[l].s, h->fork, 0));
   }
-  for (int i = 0; i < nbatch; ++i) {
+  for (int i = 0; i < nbatch && st == PD_OK; ++i) {
     const pd_scan::Lane& ln = nl == 1 ? own : h->lane[i % nl];
     Launcher L;
     L.s = ln.s;
     const int b0 = i * h->B;
     st = launch_batch(h, ln, b0, std::min(h->B, n_rot - b0), d_rots, rot0, d_peaks, d_nout, L, what);
-    if (st != PD_OK) return st;
   }
+  // the lanes join the handle's stream also after a failed launch, so that
+  // later work on that stream still follows everything already queued
   if (nl > 1)
     for (int l = 0; l < nl; ++l) {
       PD_HIP(hipEventRecord(h->lane[l].done, h->lane[l].s));
       PD_HIP(hipStreamWaitEvent(h->stream, h->lane[l].done, 0));
     }
-  return PD_OK;
+  return st;
 }
 
 // The same scan as kernel nodes of a graph: batch i on lane i % nl waits for
@@ -776,14 +790,20 @@ pd_status check_err(pd_scan* h) {
 }
 
 pd_status ensure_io(pd_scan* h, int n_rot) {
+  // a failed allocation leaves no freed pointer behind a nonzero capacity
   if ((size_t)n_rot > h->rots_cap) {
     hipFree(h->d_rots);
+    h->d_rots = nullptr;
+    h->rots_cap = 0;
     PD_HIP(hipMalloc(&h->d_rots, sizeof(double) * 9 * (size_t)n_rot));
     h->rots_cap = n_rot;
   }
   if ((size_t)n_rot > h->out_cap) {
     hipFree(h->d_peaks);
     hipFree(h->d_nout);
+    h->d_peaks = nullptr;
+    h->d_nout = nullptr;
+    h->out_cap = 0;
     PD_HIP(hipMalloc(&h->d_peaks, sizeof(PeakOut) * (size_t)n_rot * h->K));
     PD_HIP(hipMalloc(&h->d_nout, sizeof(int) * (size_t)n_rot));
     h->out_cap = n_rot;
@@ -1070,6 +1090,8 @@ pd_status pd_scan_destroy(pd_scan* h) {
   if (!h) return PD_OK;
   if (h->graph) return fail(PD_ERR_STATE, "the scan belongs to a graph: destroy the graph first");
   if (h->stream) hipStreamSynchronize(h->stream);
+  for (auto& ln : h->lane)
+    if (ln.s) hipStreamSynchronize(ln.s);
   for (auto& p : h->pending) { hipEventDestroy(p.a); hipEventDestroy(p.b); }
   free_batch(h);
   hipFree(h->tw_x); hipFree(h->tw_y); hipFree(h->tw_z); hipFree(h->tw2); hipFree(h->d_stages);
@@ -1077,7 +1099,7 @@ pd_status pd_scan_destroy(pd_scan* h) {
   hipFree(h->q); hipFree(h->w64); hipFree(h->w32);
   hipFree(h->err); hipFree(h->d_rots); hipFree(h->d_peaks); hipFree(h->d_nout);
   for (auto& ln : h->lane) {
-    if (ln.s) { hipStreamSynchronize(ln.s); hipStreamDestroy(ln.s); }
+    if (ln.s) hipStreamDestroy(ln.s);
     if (ln.done) hipEventDestroy(ln.done);
   }
   if (h->fork) hipEventDestroy(h->fork);
@@ -1340,13 +1362,13 @@ pd_status pd_scan_candidates(pd_scan* h, int n_rot, const double* rots, int rot_
   if (st != PD_OK) return st;
   st = ensure_io(h, n_rot);
   if (st != PD_OK) return st;
-  PeakOut* d_cand = nullptr;
+  CallBuf<PeakOut> d_cand;
   const size_t nc = (size_t)n_rot * h->pg.k_inp;
-  PD_HIP(hipMalloc(&d_cand, sizeof(PeakOut) * nc));
+  PD_HIP(hipMalloc(&d_cand.p, sizeof(PeakOut) * nc));
   PD_HIP(hipMemcpyAsync(h->d_rots, rots, sizeof(double) * 9 * (size_t)n_rot, hipMemcpyHostToDevice, h->stream));
-  st = run_batches(h, n_rot, h->d_rots, rot_index0, d_cand, h->d_nout, BatchOut::kCandidates);
+  st = run_batches(h, n_rot, h->d_rots, rot_index0, d_cand.p, h->d_nout, BatchOut::kCandidates);
   if (st == PD_OK) {
-    if (hipMemcpyAsync(peaks, d_cand, sizeof(PeakOut) * nc, hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+    if (hipMemcpyAsync(peaks, d_cand.p, sizeof(PeakOut) * nc, hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
         hipMemcpyAsync(n_out, h->d_nout, sizeof(int) * (size_t)n_rot, hipMemcpyDeviceToHost, h->stream) !=
             hipSuccess)
       st = fail(PD_ERR_DEVICE, "candidate copy failed");
@@ -1354,7 +1376,6 @@ pd_status pd_scan_candidates(pd_scan* h, int n_rot, const double* rots, int rot_
       st = pd_scan_sync(h);
   }
   hipStreamSynchronize(h->stream);
-  hipFree(d_cand);
   return st;
 }
 
@@ -1399,21 +1420,18 @@ pd_status pd_scan_rescore64(pd_scan* h, int n, const double* rots, const int* s,
   if (!h->receptor_ready || !h->ligand_ready) return fail(PD_ERR_STATE, "receptor or ligand not set");
   if (n == 0) return PD_OK;
   PD_HIP(hipSetDevice(h->d.device));
-  double *dr = nullptr, *de = nullptr;
-  int* ds = nullptr;
-  PD_HIP(hipMalloc(&dr, sizeof(double) * 9 * (size_t)n));
-  PD_HIP(hipMalloc(&ds, sizeof(int) * 3 * (size_t)n));
-  PD_HIP(hipMalloc(&de, sizeof(double) * (size_t)n));
-  PD_HIP(hipMemcpy(dr, rots, sizeof(double) * 9 * (size_t)n, hipMemcpyHostToDevice));
-  PD_HIP(hipMemcpy(ds, s, sizeof(int) * 3 * (size_t)n, hipMemcpyHostToDevice));
-  hipLaunchKernelGGL(k_rescore, dim3(n), dim3(256), 0, h->stream, h->g, dr, ds, h->q, h->n_lig, h->w64,
-                     h->R64, de);
+  CallBuf<double> dr, de;
+  CallBuf<int> ds;
+  PD_HIP(hipMalloc(&dr.p, sizeof(double) * 9 * (size_t)n));
+  PD_HIP(hipMalloc(&ds.p, sizeof(int) * 3 * (size_t)n));
+  PD_HIP(hipMalloc(&de.p, sizeof(double) * (size_t)n));
+  PD_HIP(hipMemcpy(dr.p, rots, sizeof(double) * 9 * (size_t)n, hipMemcpyHostToDevice));
+  PD_HIP(hipMemcpy(ds.p, s, sizeof(int) * 3 * (size_t)n, hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_rescore, dim3(n), dim3(256), 0, h->stream, h->g, dr.p, ds.p, h->q, h->n_lig, h->w64,
+                     h->R64, de.p);
   PD_HIP(hipGetLastError());
-  PD_HIP(hipMemcpyAsync(energies, de, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost, h->stream));
+  PD_HIP(hipMemcpyAsync(energies, de.p, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost, h->stream));
   PD_HIP(hipStreamSynchronize(h->stream));
-  hipFree(dr);
-  hipFree(ds);
-  hipFree(de);
   return PD_OK;
 }
 
@@ -1450,18 +1468,17 @@ pd_status pd_scan_correlation_grid(pd_scan* h, const double* rot, float* out) {
   PD_HIP(hipSetDevice(h->d.device));
   pd_status st = ensure_io(h, 1);
   if (st != PD_OK) return st;
-  float* dgrid = nullptr;
-  PD_HIP(hipMalloc(&dgrid, sizeof(float) * h->N));
+  CallBuf<float> dgrid;
+  PD_HIP(hipMalloc(&dgrid.p, sizeof(float) * h->N));
   PD_HIP(hipMemcpy(h->d_rots, rot, sizeof(double) * 9, hipMemcpyHostToDevice));
   pd_scan::Lane ln = h->lane[0];
   ln.s = h->stream;
   Launcher L;
   L.s = h->stream;
-  st = launch_fft_path(h, ln, 1, h->d_rots, dgrid, L);
+  st = launch_fft_path(h, ln, 1, h->d_rots, dgrid.p, L);
   if (st != PD_OK) return st;
-  PD_HIP(hipMemcpyAsync(out, dgrid, sizeof(float) * h->N, hipMemcpyDeviceToHost, h->stream));
+  PD_HIP(hipMemcpyAsync(out, dgrid.p, sizeof(float) * h->N, hipMemcpyDeviceToHost, h->stream));
   PD_HIP(hipStreamSynchronize(h->stream));
-  hipFree(dgrid);
   return check_err(h);
 }
 

@@ -261,6 +261,7 @@ pd_status build_maps(pd_prop* h, int m, int n, const double* pos) {
     hipFree(h->surf);
     h->atoms = nullptr;
     h->surf = nullptr;
+    h->atoms_cap = 0;  // a failed allocation leaves no capacity behind
     PP_HIP(hipMalloc(&h->atoms, sizeof(double) * 3 * (size_t)n));
     PP_HIP(hipMalloc(&h->surf, sizeof(float) * (size_t)n));
     h->atoms_cap = n;
