@@ -46,9 +46,10 @@ struct FastSet {
   int n;
   LigFn lig;
   InvFn inv;
+  int lig_threads;  // block size of the ligand kernel (lig_plane_threads); the inverse takes kPlaneThreads
 };
 #define PD_FAST(N) \
-  FastSet { N, k_lig_plane_fwd_fast<N, N>, k_plane_inv_topk_fast<N, N> }
+  FastSet { N, k_lig_plane_fwd_fast<N, N>, k_plane_inv_topk_fast<N, N>, lig_plane_threads(N) }
 const FastSet kFast[] = {PD_FAST(32),  PD_FAST(48),  PD_FAST(64),  PD_FAST(96),
                          PD_FAST(120), PD_FAST(128), PD_FAST(144), PD_FAST(160)};
 #undef PD_FAST
@@ -615,7 +616,7 @@ pd_status launch_fft_path(pd_scan* h, const pd_scan::Lane& ln, int nb, const dou
                            dim3(kSplitThreads), split_col_lds(h), g, h->py, ln.L1, g.p[0], g.pp,
                            g.lo[0] - g.pl, g.lo[1], g.lo[1] + g.p[1]);
     } else if (f)
-      L(f->lig, dim3((g.p[0] + 15) / 16 * 16, nb), dim3(kPlaneThreads), plane_lds(h), g,
+      L(f->lig, dim3((g.p[0] + 15) / 16 * 16, nb), dim3(f->lig_threads), plane_lds(h), g,
                          h->tw_y, h->tw_z, h->tw2, ln.cells, h->n_lig, h->w32, ln.L1);
     else
       L(k_lig_plane_fwd, dim3(g.p[0], nb), dim3(kPlaneThreads), plane_lds(h),
@@ -1014,12 +1015,15 @@ pd_status pd_scan_create(const pd_scan_desc* desc, pd_scan** out) {
     h->pz = FftPlan{g.m2, nz, h->d_stages + nx + ny, h->tw_z};
   }
   // A whole-plane grid whose plane takes more than half of LDS (one block per
-  // CU: 144^3, 160^3) runs the ligand pass as the split row and column
-  // kernels, which measured twice as fast there (160^3: 32.9 -> 15.4 us per
-  // rotation); its inverse keeps the whole-plane kernel.
+  // CU) runs the ligand pass as the split row and column kernels when it has
+  // no specialised whole-plane kernel: they beat the generic deposit + FFT
+  // kernel (160^3: 32.9 -> 15.4 us per rotation).  The specialised kernels,
+  // which sum each row's z spectrum directly, beat the split kernels at 144^3
+  // (10.2 against 15.3 us) and 160^3 (8.2 against 16.4 us; pipeline 24.1k ->
+  // 30.2k rotations/s) and run there.  The inverse keeps the whole-plane kernel.
   h->rows_ok = !h->split && plan_fits(h->hz, g.zrows, kSplitThreads) && plan_fits(h->hy, kSplitCols, kSplitThreads) &&
                split_row_lds(h) <= kMaxLds && split_col_lds(h) <= kMaxLds;
-  h->rows_auto = h->rows_ok && 2 * plane_lds(h) > kMaxLds;
+  h->rows_auto = h->rows_ok && 2 * plane_lds(h) > kMaxLds && !find_fast(g.n0, g.n1, g.n2);
   h->lig_rows = h->rows_auto;
   if (h->rows_ok) {
     if ((st = set_lds((const void*)k_lig_rows, split_row_lds(h))) != PD_OK ||
