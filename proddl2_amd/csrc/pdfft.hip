@@ -46,10 +46,10 @@ struct FastSet {
   int n;
   LigFn lig;
   InvFn inv;
-  int lig_threads;  // block size of the ligand kernel (lig_plane_threads); the inverse takes kPlaneThreads
+  int threads;  // block size of both kernels (their template default, plane_threads)
 };
 #define PD_FAST(N) \
-  FastSet { N, k_lig_plane_fwd_fast<N, N>, k_plane_inv_topk_fast<N, N>, lig_plane_threads(N) }
+  FastSet { N, k_lig_plane_fwd_fast<N, N>, k_plane_inv_topk_fast<N, N>, plane_threads(N) }
 const FastSet kFast[] = {PD_FAST(32),  PD_FAST(48),  PD_FAST(64),  PD_FAST(96),
                          PD_FAST(120), PD_FAST(128), PD_FAST(144), PD_FAST(160)};
 #undef PD_FAST
@@ -616,7 +616,7 @@ pd_status launch_fft_path(pd_scan* h, const pd_scan::Lane& ln, int nb, const dou
                            dim3(kSplitThreads), split_col_lds(h), g, h->py, ln.L1, g.p[0], g.pp,
                            g.lo[0] - g.pl, g.lo[1], g.lo[1] + g.p[1]);
     } else if (f)
-      L(f->lig, dim3((g.p[0] + 15) / 16 * 16, nb), dim3(f->lig_threads), plane_lds(h), g,
+      L(f->lig, dim3((g.p[0] + 15) / 16 * 16, nb), dim3(f->threads), plane_lds(h), g,
                          h->tw_y, h->tw_z, h->tw2, ln.cells, h->n_lig, h->w32, ln.L1);
     else
       L(k_lig_plane_fwd, dim3(g.p[0], nb), dim3(kPlaneThreads), plane_lds(h),
@@ -654,7 +654,7 @@ pd_status launch_fft_path(pd_scan* h, const pd_scan::Lane& ln, int nb, const dou
         L(k_inv_rows_topk, dim3(g.n0, nb), dim3(kSplitThreads), split_row_lds(h), g,
                            h->pz, h->tw2, ln.S1, scale, h->K, ln.cval, ln.coff, grid_out);
     } else if (f)
-      L(f->inv, dim3(nb, g.n0), dim3(kPlaneThreads), plane_lds(h), g, h->tw_y,
+      L(f->inv, dim3(nb, g.n0), dim3(f->threads), plane_lds(h), g, h->tw_y,
                          h->tw_z, h->tw2, ln.S1, scale, h->K, ln.cval, ln.coff, grid_out, ln.thr);
     else
       L(k_plane_inv_topk, dim3(g.n0, nb), dim3(kPlaneThreads), plane_lds(h),

@@ -6,6 +6,8 @@
 // line, one LDS exchange per transform, inputs loaded straight from HBM into
 // registers and outputs stored straight from registers.
 #pragma once
+#include <type_traits>
+
 #include "reg_fft.hpp"
 #include "scan_kernels.hpp"
 
@@ -19,6 +21,12 @@
 #ifndef PD_LIG_WAVES
 #define PD_LIG_WAVES PD_PLANE_WAVES
 #endif
+
+// N = A * B with a given B.
+template <int N, int BB>
+struct SplitB {
+  static constexpr int B = BB, A = N / BB, J = (A + BB - 1) / BB;
+};
 
 // N = A * B with B lanes per line; B = 0 marks an unsupported edge.
 template <int N>
@@ -521,15 +529,14 @@ struct LigList {
   static constexpr int value = kFit >= 384 ? (kFit < 1024 ? kFit : 1024) : 1024;
 };
 // ---------------------------------------------------------------------------
-// Threads per block of the whole-plane ligand pass: 1,024 where the plane
+// Threads per block of the two whole-plane kernels: 1,024 where the plane
 // takes more than 80 KB of LDS (144, 160: one block per CU either way), so
 // that each thread holds half as many values and the CU runs four waves per
 // SIMD instead of two (ligand pass 144^3: 10.2 -> 9.3 us per rotation, 160^3:
-// 8.2 -> 7.3, same bits).  The inverse pass stays at 512: at 1,024 threads it
-// spills (148 bytes per lane) and measured 2-3% slower.
-constexpr int lig_plane_threads(int n) { return n >= 144 ? 1024 : 512; }
+// 8.2 -> 7.3, same bits).  The launch must use the same count.
+constexpr int plane_threads(int n) { return n >= 144 ? 1024 : 512; }
 
-template <int N1, int N2, int NT = lig_plane_threads(N1)>
+template <int N1, int N2, int NT = plane_threads(N1)>
 __global__ __launch_bounds__(NT, PD_LIG_WAVES) void k_lig_plane_fwd_fast(ScanGeom g, const float2* __restrict__ twN1,
                                                             const float2* __restrict__ twM2,
                                                             const float2* __restrict__ tw2,
@@ -761,7 +768,7 @@ __global__ __launch_bounds__(NT, PD_LIG_WAVES) void k_lig_plane_fwd_fast(ScanGeo
 // ---------------------------------------------------------------------------
 // Inverse plane pass + per-plane top-K.  grid (nrot, n0), block NT.
 // ---------------------------------------------------------------------------
-template <int N1, int N2, int NT = 512>
+template <int N1, int N2, int NT = plane_threads(N1)>
 __global__ __launch_bounds__(NT, PD_PLANE_WAVES) void k_plane_inv_topk_fast(ScanGeom g, const float2* __restrict__ twN1,
                                                              const float2* __restrict__ twM2,
                                                              const float2* __restrict__ tw2,
@@ -772,10 +779,21 @@ __global__ __launch_bounds__(NT, PD_PLANE_WAVES) void k_plane_inv_topk_fast(Scan
                                                              unsigned* __restrict__ thr) {
   constexpr int M2 = N2 / 2, H2 = M2 + 1, LS = H2 | 1;
   using SY = Split<N1>;
-  using SZ = Split<M2>;
+  // z pass: 4 lanes per row at 1,024 threads (144, 160: one round of N1 rows
+  // instead of two with 8 lanes, so fewer values per lane)
+  using SZ = std::conditional_t<(NT >= 1024 && M2 % 4 == 0 && N1 * 4 <= NT), SplitB<M2, 4>, Split<M2>>;
   constexpr int A1 = SY::A, B1 = SY::B, J1 = SY::J;
   constexpr int A2 = SZ::A, B2 = SZ::B, J2 = SZ::J;
-  constexpr int LY = NT / B1, RY = (M2 + LY - 1) / LY;  // y columns: M2 (0 and M2 packed)
+  constexpr int LY = NT / B1;
+  // Columns 0 and M2 are Hermitian along y (their transforms are real).  With
+  // fewer lane groups than H2 columns they travel packed as one complex column
+  // X[.][0] + i X[.][M2] in slot 0, and the z pass unpacks real and imaginary
+  // parts; with LY >= H2 (1,024 threads at 144 and 160) column M2 has lanes of
+  // its own and both are transformed alone (no packed loads, which would hold
+  // 2 A1 more registers per lane).
+  constexpr bool packed = LY < H2;
+  constexpr int NC = packed ? M2 : H2;  // columns the y pass transforms
+  constexpr int RY = (NC + LY - 1) / LY;
   constexpr int LZ = NT / B2, RZ = (N1 + LZ - 1) / LZ;
   extern __shared__ float2 buf[];
   const int tid = threadIdx.x;
@@ -789,14 +807,11 @@ __global__ __launch_bounds__(NT, PD_PLANE_WAVES) void k_plane_inv_topk_fast(Scan
   // atomics do not update.  It is issued after the plane's loads, so that its
   // latency overlaps theirs; the barriers of the y pass publish it.
   __shared__ uint32_t s_thr;
-  // y pass: backward FFT along y of every column.  Columns 0 and M2 are
-  // Hermitian along y (their transforms are real): they travel packed as one
-  // complex column X[.][0] + i X[.][M2] in column slot 0, so the pass covers M2
-  // columns and the z pass unpacks real and imaginary parts.  Every lane loads
-  // column M2 of its rows with an ordinary vector load issued with the others:
-  // a divergent, scalar or LDS-staged load costs an extra memory round trip
-  // before the transform.  The empty asm consumes those values in every lane,
-  // so the compiler cannot sink the loads into the column-0 branch.
+  // y pass: backward FFT along y of every column (NC of them).  Packed, every
+  // lane loads column M2 of its rows with an ordinary vector load issued with
+  // the others: a divergent, scalar or LDS-staged load costs an extra memory
+  // round trip before the transform.  The empty asm consumes those values in
+  // every lane, so the compiler cannot sink the loads into the column-0 branch.
   {
     const int gy = tid / LY, cy = tid % LY;
     // LY is a multiple of 64: gy is wave-uniform, and so are the twiddle reads
@@ -806,18 +821,23 @@ __global__ __launch_bounds__(NT, PD_PLANE_WAVES) void k_plane_inv_topk_fast(Scan
 #pragma unroll
     for (int r = 0; r < RY; ++r) {
       const int col = r * LY + cy;
-      if (col < M2) {
-        float2 ym[A1];
+      if (col < NC) {
+        if constexpr (packed) {
+          float2 ym[A1];
 #pragma unroll
-        for (int m = 0; m < A1; ++m) {
-          const int row = gy + B1 * m;
-          v[r][m] = src[row * H2 + col];
-          ym[m] = src[row * H2 + M2];
-        }
+          for (int m = 0; m < A1; ++m) {
+            const int row = gy + B1 * m;
+            v[r][m] = src[row * H2 + col];
+            ym[m] = src[row * H2 + M2];
+          }
 #pragma unroll
-        for (int m = 0; m < A1; ++m) {
-          asm volatile("" ::"v"(ym[m].x), "v"(ym[m].y));
-          if (col == 0) v[r][m] = make_float2(v[r][m].x - ym[m].y, v[r][m].y + ym[m].x);
+          for (int m = 0; m < A1; ++m) {
+            asm volatile("" ::"v"(ym[m].x), "v"(ym[m].y));
+            if (col == 0) v[r][m] = make_float2(v[r][m].x - ym[m].y, v[r][m].y + ym[m].x);
+          }
+        } else {
+#pragma unroll
+          for (int m = 0; m < A1; ++m) v[r][m] = src[(gy + B1 * m) * H2 + col];
         }
       }
     }
@@ -825,7 +845,7 @@ __global__ __launch_bounds__(NT, PD_PLANE_WAVES) void k_plane_inv_topk_fast(Scan
 #pragma unroll
     for (int r = 0; r < RY; ++r) {
       const int col = r * LY + cy;
-      if (col < M2) {
+      if (col < NC) {
         four_step_phase1<A1, B1, +1>(v[r], gyu, twN1);
 #pragma unroll
         for (int k1 = 0; k1 < A1; ++k1) buf[(k1 * B1 + gy) * LS + col] = v[r][k1];
@@ -839,7 +859,7 @@ __global__ __launch_bounds__(NT, PD_PLANE_WAVES) void k_plane_inv_topk_fast(Scan
 #pragma unroll
       for (int j = 0; j < J1; ++j) {
         const int k1 = gy + B1 * j;
-        if (col < M2 && k1 < A1)
+        if (col < NC && k1 < A1)
 #pragma unroll
           for (int q = 0; q < B1; ++q) w[r][j][q] = buf[(k1 * B1 + q) * LS + col];
       }
@@ -851,7 +871,7 @@ __global__ __launch_bounds__(NT, PD_PLANE_WAVES) void k_plane_inv_topk_fast(Scan
 #pragma unroll
       for (int j = 0; j < J1; ++j) {
         const int k1 = gy + B1 * j;
-        if (col < M2 && k1 < A1) {
+        if (col < NC && k1 < A1) {
           rdft<B1, +1>(w[r][j]);
 #pragma unroll
           for (int k2 = 0; k2 < B1; ++k2) buf[(k1 + A1 * k2) * LS + col] = w[r][j][k2];
@@ -875,8 +895,11 @@ __global__ __launch_bounds__(NT, PD_PLANE_WAVES) void k_plane_inv_topk_fast(Scan
         for (int m = 0; m < A2; ++m) {
           const int k = gz + B2 * m, ku = gzu + B2 * m;
           float2 xk = buf[row * LS + k], xm;
-          if (k == 0) {  // packed column: X[0] = Re, X[M2] = Im (both real)
-            xm = make_float2(xk.y, 0.f);
+          if (k == 0) {  // X[0] and X[M2] are real: packed as Re and Im, or in slots 0 and M2
+            if constexpr (packed)
+              xm = make_float2(xk.y, 0.f);
+            else
+              xm = make_float2(buf[row * LS + M2].x, 0.f);
             xk.y = 0.f;
           } else {
             xm = buf[row * LS + M2 - k];
