@@ -13,7 +13,8 @@
 // Receptor:        k_rec_rows + k_cols_fft<-1>, then the usual k_col_fwd.
 // N1 = N2 = 256:   each plane pass in one 1,024-thread kernel that holds a
 //                  plane on chip in registers and LDS: k_lig_plane_split
-//                  (ligand boxes of up to 128 rows) and k_inv_plane_topk_split.
+//                  (ligand boxes of up to 128 rows; each row's z spectrum by
+//                  direct summation over its atoms) and k_inv_plane_topk_split.
 //
 // The same transforms (r2c along z, FFT along y, and their inverses; the FFT
 // factorisations differ, so results agree to rounding) and the same selection
@@ -725,21 +726,25 @@ __global__ __launch_bounds__(1024) void k_inv_plane_topk_split(ScanGeom g, const
 // Ligand plane pass in one kernel for N1 = 256 and N2 = 256 with at most 128
 // box rows: the plane's spectrum is written once, as whole rows.
 // grid (p0 padded to 16, nrot, T), block 1024 = 16 waves; one term per block:
-//   1. deposit the plane's atoms into the box rows (real rows in LDS, row
-//      stride LS complex);
-//   2. packed half-length forward z FFT of each box row (register four-step,
-//      one row per 8 lanes, exchange in place);
-//   3. r2c post-step in place (X[k] and X[M2 - k] from Z[k] and Z[M2 - k]);
-//   4. the lanes load their column sets (lane c, wave r: kz = c + 64 s, rows
+//   1. the z spectrum of each box row by direct summation over the row's
+//      atoms, X[row][kz] = sum_j w_j exp(-2 pi i kz z_j / N2) for kz = 0..M2,
+//      in ascending atom order (the plane's atoms are listed by row, then
+//      each row is put in atom order), one row per wave, lane c taking
+//      columns c + 64 q; X[0] and X[M2] are real.  A plane with more atoms
+//      than the list holds is summed from the atom cells directly (same
+//      order, slower);
+//   2. the lanes load their column sets (lane c, wave r: kz = c + 64 s, rows
 //      r + 16 m, zero outside the box), forward y FFT four-step 16 x 16 with
 //      one LDS exchange per set; a wave's stores cover 512 contiguous bytes
 //      of a row (plane-major) or 32-byte pieces of the column pass's
 //      interleaved order (g.il), whose 16-plane groups run on one XCD so that
 //      its L2 merges the pieces into whole lines.  Column kz = M2 is
 //      transformed by lanes 0..15 of wave 0 alone.
-// LDS: 128 * LS complex (the box rows, then the y exchange).
+// LDS: 128 * LS complex (the box rows, then the y exchange), the twiddles and
+// the atom list.
 // ---------------------------------------------------------------------------
 constexpr int kPlaneSplitRows = 128;  // box rows the one-kernel ligand pass holds
+constexpr int kPlaneSplitList = 1536;  // atoms of one plane the one-kernel ligand pass lists
 
 template <int N1, int N2>
 __global__ __launch_bounds__(1024) void k_lig_plane_split(ScanGeom g, const float2* __restrict__ twN1,
@@ -752,13 +757,15 @@ __global__ __launch_bounds__(1024) void k_lig_plane_split(ScanGeom g, const floa
   static_assert(A1 * B1 == N1, "y split");
   constexpr int M2 = N2 / 2, H2 = M2 + 1, LS = H2 | 1, NS = M2 / 64;
   static_assert(M2 % 64 == 0, "whole column sets");
-  using SZ = Split<M2>;
-  constexpr int A2 = SZ::A, B2 = SZ::B, J2 = SZ::J;
-  constexpr int LZ = NT / B2;
-  static_assert(LZ == kPlaneSplitRows && LZ % 64 == 0, "one z pass over the box rows");
+  (void)twM2;
   extern __shared__ float2 buf[];
-  float* fb = reinterpret_cast<float*>(buf);
   __shared__ float2 s_nyq[N1];
+  __shared__ float2 s_tw[N2];
+  __shared__ int s_row[kPlaneSplitRows + 1];  // row r's atoms: [s_row[r], s_row[r + 1])
+  __shared__ int s_cur[kPlaneSplitRows];
+  __shared__ int s_z[kPlaneSplitList];
+  __shared__ int s_ra[kPlaneSplitList];
+  __shared__ float s_w[kPlaneSplitList];
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   // blocks go to the 8 XCDs round-robin in launch order: block L (on XCD
@@ -772,69 +779,91 @@ __global__ __launch_bounds__(1024) void k_lig_plane_split(ScanGeom g, const floa
   if (p >= g.p[0]) return;  // padding plane: the whole block leaves before any barrier
   const int i0 = g.lo[0] + p, lo1 = g.lo[1], p1 = g.p[1];
   const int plane = N1 * N2;
-  const int gz = tid / LZ, rz = tid % LZ;
-  const int gzu = __builtin_amdgcn_readfirstlane(gz);
-  for (int e = tid; e < p1 * LS; e += NT) buf[e] = make_float2(0.f, 0.f);
+  const int* cb = cells + (size_t)b * n_atoms;
+  const float* wt = w + (size_t)t * n_atoms;
+  for (int i = tid; i < N2; i += NT) s_tw[i] = tw2[i];
+  for (int i = tid; i <= p1; i += NT) s_row[i] = 0;
   __syncthreads();
-  {
-    const int* cb = cells + (size_t)b * n_atoms;
-    const float* wt = w + (size_t)t * n_atoms;
+  // box row * N2 + z of atom j if it lies in this plane, else -1
+  auto key_of = [&](int j) -> int {
+    const int off = cb[j];
+    if (off < 0 || off / plane != i0) return -1;
+    const int r = off - i0 * plane;
+    const int y = r / N2;
+    return (y - lo1) * N2 + (r - y * N2);
+  };
+  for (int j = tid; j < n_atoms; j += NT) {
+    const int k = key_of(j);
+    if (k >= 0) atomicAdd(&s_row[k / N2 + 1], 1);
+  }
+  __syncthreads();
+  if (tid == 0)
+    for (int i = 1; i <= p1; ++i) s_row[i] += s_row[i - 1];
+  __syncthreads();
+  const bool listed = s_row[p1] <= kPlaneSplitList;
+  if (listed) {
+    for (int i = tid; i < p1; i += NT) s_cur[i] = s_row[i];
+    __syncthreads();
     for (int j = tid; j < n_atoms; j += NT) {
-      const int off = cb[j];
-      if (off >= 0 && off / plane == i0) {
-        const int r = off - i0 * plane;
-        const int i1 = r / N2, i2 = r - i1 * N2;
-        atomicAdd(&fb[(i1 - lo1) * 2 * LS + i2], wt[j]);
+      const int k = key_of(j);
+      if (k >= 0) {
+        const int r = k / N2;
+        const int pos = atomicAdd(&s_cur[r], 1);
+        s_z[pos] = k - r * N2;
+        s_ra[pos] = j;
+        s_w[pos] = wt[j];
       }
     }
+    __syncthreads();
+    // each row in atom order (insertion sort of its few entries)
+    for (int r = tid; r < p1; r += NT) {
+      const int a0 = s_row[r], a1 = s_row[r + 1];
+      for (int i = a0 + 1; i < a1; ++i) {
+        const int ja = s_ra[i], za = s_z[i];
+        const float wa = s_w[i];
+        int q = i - 1;
+        while (q >= a0 && s_ra[q] > ja) {
+          s_ra[q + 1] = s_ra[q];
+          s_z[q + 1] = s_z[q];
+          s_w[q + 1] = s_w[q];
+          --q;
+        }
+        s_ra[q + 1] = ja;
+        s_z[q + 1] = za;
+        s_w[q + 1] = wa;
+      }
+    }
+    __syncthreads();
   }
-  __syncthreads();
-  // z pass: row rz of the box, 8 lanes
   {
-    const bool rok = rz < p1;
-    float2 z[A2];
-    if (rok) {
+    constexpr int NQ = (H2 + 63) / 64;  // columns per lane
+    for (int r = wave; r < p1; r += NW) {
+      float2 a[NQ];
 #pragma unroll
-      for (int m = 0; m < A2; ++m) z[m] = buf[rz * LS + gz + B2 * m];
-      four_step_phase1<A2, B2, -1>(z, gzu, twM2);
-    }
-    __syncthreads();
-    if (rok)
+      for (int q = 0; q < NQ; ++q) a[q] = make_float2(0.f, 0.f);
+      auto add = [&](int z, float wv) {
 #pragma unroll
-      for (int k1 = 0; k1 < A2; ++k1) buf[rz * LS + k1 * B2 + gz] = z[k1];
-    __syncthreads();
-    float2 u[J2][B2];
-#pragma unroll
-    for (int j = 0; j < J2; ++j) {
-      const int k1 = gz + B2 * j;
-      if (rok && k1 < A2)
-#pragma unroll
-        for (int q = 0; q < B2; ++q) u[j][q] = buf[rz * LS + k1 * B2 + q];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int j = 0; j < J2; ++j) {
-      const int k1 = gz + B2 * j;
-      if (rok && k1 < A2) {
-        rdft<B2, -1>(u[j]);
-#pragma unroll
-        for (int k2 = 0; k2 < B2; ++k2) buf[rz * LS + k1 + A2 * k2] = u[j][k2];
+        for (int q = 0; q < NQ; ++q) {
+          const int c = lane + 64 * q;
+          const float2 ph = s_tw[(c * z) % N2];
+          a[q].x = fmaf(wv, ph.x, a[q].x);
+          a[q].y = fmaf(wv, c == 0 || c == M2 ? 0.f : ph.y, a[q].y);
+        }
+      };
+      if (listed) {
+        const int a0 = s_row[r], a1 = s_row[r + 1];
+        for (int i = a0; i < a1; ++i) add(s_z[i], s_w[i]);
+      } else {
+        for (int j = 0; j < n_atoms; ++j) {
+          const int k = key_of(j);
+          if (k >= 0 && k / N2 == r) add(k - r * N2, wt[j]);
+        }
       }
-    }
-    __syncthreads();
-  }
-  // r2c post-step in place: X[0] = Re Z0 + Im Z0 and X[M2] = Re Z0 - Im Z0
-  // (both real), X[k] and X[M2 - k] from the pair Z[k], Z[M2 - k]
-  constexpr int NP = M2 / 2 + 1;
-  for (int e = tid; e < p1 * NP; e += NT) {
-    const int rr = e / NP, k = e - rr * NP;
-    const float2 zk = buf[rr * LS + k], zm = buf[rr * LS + (M2 - k) % M2];
-    if (k == 0) {
-      buf[rr * LS] = make_float2(zk.x + zk.y, 0.f);
-      buf[rr * LS + M2] = make_float2(zk.x - zk.y, 0.f);
-    } else {
-      buf[rr * LS + k] = r2c_point(zk, zm, tw2[k]);
-      if (k != M2 - k) buf[rr * LS + M2 - k] = r2c_point(zm, zk, tw2[M2 - k]);
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        const int c = lane + 64 * q;
+        if (c <= M2) buf[r * LS + c] = a[q];
+      }
     }
   }
   __syncthreads();
