@@ -41,11 +41,16 @@ def test_zero_rotations():
         assert cand.shape[0] == 0 and ncand.shape == (0,)
 
 
-def test_one_rotation_and_one_per_launch_same_bits(oracle):
-    print("case one rotation / batch 1", flush=True)
-    cx, rt, w, R = build_case(oracle, N48, 1.5, 500, 150, 2, seed=62)
+# 160^3 runs the 1,024-thread whole-plane kernels (16 waves per block)
+EDGE_CASES = [(N48, 1.5, 500, 150), ((160, 160, 160), 1.2, 2000, 500)]
+
+
+@pytest.mark.parametrize("n,h,n_rec,n_lig", EDGE_CASES)
+def test_one_rotation_and_one_per_launch_same_bits(oracle, n, h, n_rec, n_lig):
+    print(f"case one rotation / batch 1, n={n}", flush=True)
+    cx, rt, w, R = build_case(oracle, n, h, n_rec, n_lig, 2, seed=62)
     rots = synth.rotation_set(7, 63)
-    with make_scan(N48, 1.5, 2, 6, cx, rt, w, batch=64) as sc:
+    with make_scan(n, h, 2, 6, cx, rt, w, batch=64) as sc:
         ref, nref = sc.run(rots)
         for i in range(len(rots)):
             p1, n1 = sc.run(rots[i:i + 1], rot_index0=i)
@@ -55,16 +60,18 @@ def test_one_rotation_and_one_per_launch_same_bits(oracle):
         assert np.array_equal(n, nref) and np.array_equal(p.view(np.uint8), ref.view(np.uint8))
 
 
-def test_largest_maxntrans_matches_oracle(oracle):
+@pytest.mark.parametrize("n,h,n_rec,n_lig", [(N48, 1.5, 700, 220), ((160, 160, 160), 1.2, 2000, 500)])
+def test_largest_maxntrans_matches_oracle(oracle, n, h, n_rec, n_lig):
+    """At 160^3 the inverse pass's block merge takes 16 waves x 32 keys, eight per lane."""
     K = 32
-    print(f"case K={K}", flush=True)
-    cx, rt, w, R = build_case(oracle, N48, 1.5, 700, 220, 2, seed=64)
+    print(f"case K={K}, n={n}", flush=True)
+    cx, rt, w, R = build_case(oracle, n, h, n_rec, n_lig, 2, seed=64)
     S = oracle.receptor_spectra(R)
     rots = synth.rotation_set(5, 65)
-    tol = tolerance(R, w, N48)
-    nout_o, off_o, s_o, val_o = oracle.scan(N48, 1.5, S, cx.ligand.pos, w, rots, K=K + 1, max_val=1e16,
+    tol = tolerance(R, w, n)
+    nout_o, off_o, s_o, val_o = oracle.scan(n, h, S, cx.ligand.pos, w, rots, K=K + 1, max_val=1e16,
                                             n_threads=4)
-    with make_scan(N48, 1.5, 2, K, cx, rt, w, batch=4) as sc:
+    with make_scan(n, h, 2, K, cx, rt, w, batch=4) as sc:
         peaks, nout = sc.run(rots)
     checked = compare_rotation_peaks(peaks, nout, val_o, off_o, s_o, tol, K)
     print(f"  checked {checked} of {len(rots) * K} ranks exactly", flush=True)
