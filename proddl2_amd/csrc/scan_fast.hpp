@@ -786,11 +786,13 @@ __global__ __launch_bounds__(NT, PD_PLANE_WAVES) void k_plane_inv_topk_fast(Scan
   constexpr int A2 = SZ::A, B2 = SZ::B, J2 = SZ::J;
   constexpr int LY = NT / B1;
   // Columns 0 and M2 are Hermitian along y (their transforms are real).  With
-  // fewer lane groups than H2 columns they travel packed as one complex column
-  // X[.][0] + i X[.][M2] in slot 0, and the z pass unpacks real and imaginary
-  // parts; with LY >= H2 (1,024 threads at 144 and 160) column M2 has lanes of
-  // its own and both are transformed alone (no packed loads, which would hold
-  // 2 A1 more registers per lane).
+  // fewer lane groups than H2 columns (128 at 512 threads) they travel packed
+  // as one complex column X[.][0] + i X[.][M2] in slot 0, and the z pass
+  // unpacks real and imaginary parts.  With LY >= H2 (up to 120 at 512
+  // threads, 144 and 160 at 1,024) column M2 has lanes of its own that would
+  // otherwise idle, and both are transformed alone: no packed loads, which
+  // hold 2 A1 more registers per lane and wait in every lane (inverse pass
+  // 96^3: 2.12 -> 1.94 us per rotation, 120^3: 3.27 -> 2.91).
   constexpr bool packed = LY < H2;
   constexpr int NC = packed ? M2 : H2;  // columns the y pass transforms
   constexpr int RY = (NC + LY - 1) / LY;
