@@ -546,7 +546,18 @@ __global__ __launch_bounds__(NT, PD_LIG_WAVES) void k_lig_plane_fwd_fast(ScanGeo
   constexpr int M2 = N2 / 2, H2 = M2 + 1, LS = H2 | 1;
   using SY = Split<N1>;
   constexpr int A1 = SY::A, B1 = SY::B, J1 = SY::J;
-  constexpr int LY = NT / B1, RY = (M2 + LY - 1) / LY;  // y columns: M2 (0 and M2 packed)
+  constexpr int LY = NT / B1;
+  // Columns 0 and M2 are real along y.  With fewer lane groups than H2
+  // columns (128 at 512 threads) they travel packed as one complex column
+  // X[.][0] + i X[.][M2] in slot 0 and are split by Hermitian symmetry after
+  // the y pass; with LY >= H2 (up to 120 at 512 threads, 160 at 1,024) each
+  // has lanes of its own that would otherwise idle, and no split follows
+  // (ligand pass 96^3: 2.20 -> 2.06 us per rotation, 120^3: 2.71 -> 2.53,
+  // 160^3: 7.21 -> 6.90).  At 144 the unpacked pass spills more (100 against
+  // 72 bytes per lane) and measured 3% slower, so 144 stays packed.
+  constexpr bool packed = LY < H2 || N1 == 144;
+  constexpr int NC = packed ? M2 : H2;  // columns the y pass transforms
+  constexpr int RY = (NC + LY - 1) / LY;
   extern __shared__ float2 buf[];
   const int tid = threadIdx.x;
   // Blocks go to the 8 XCDs round-robin in launch order.  The 16 planes of one
@@ -641,11 +652,11 @@ __global__ __launch_bounds__(NT, PD_LIG_WAVES) void k_lig_plane_fwd_fast(ScanGeo
     }
     __syncthreads();
   }
-  constexpr int NQ = (M2 + 63) / 64;  // columns per lane in the summation
+  constexpr int NQ = (NC + 63) / 64;  // columns per lane in the summation
   for (int t = 0; t < g.T; ++t) {
     // Box rows of term t into buf, one row per wave at a time, lane c taking
-    // columns c + 64 q: slot c in [1, M2) holds X[c], slot 0 holds
-    // X[0] + i X[M2] (both real), the packed column the y pass takes.
+    // columns c + 64 q: slot c in [1, M2) holds X[c]; slot 0 holds
+    // X[0] + i X[M2] (both real) when packed, else X[0] and slot M2 X[M2].
     const float* wt = w + (size_t)t * n_atoms;
     for (int r = wave; r < p1; r += NT / 64) {
       float2 a[NQ];
@@ -656,7 +667,7 @@ __global__ __launch_bounds__(NT, PD_LIG_WAVES) void k_lig_plane_fwd_fast(ScanGeo
         for (int q = 0; q < NQ; ++q) {
           const int c = lane + 64 * q;
           const float2 ph = s_tw[(c * z) % N2];
-          const float py = c == 0 ? s_tw[(M2 * z) % N2].x : ph.y;
+          const float py = packed ? (c == 0 ? s_tw[(M2 * z) % N2].x : ph.y) : (c == 0 || c == M2 ? 0.f : ph.y);
           a[q].x = fmaf(wv, ph.x, a[q].x);
           a[q].y = fmaf(wv, py, a[q].y);
         }
@@ -673,20 +684,18 @@ __global__ __launch_bounds__(NT, PD_LIG_WAVES) void k_lig_plane_fwd_fast(ScanGeo
 #pragma unroll
       for (int q = 0; q < NQ; ++q) {
         const int c = lane + 64 * q;
-        if (c < M2) buf[(lo1 + r) * LS + c] = a[q];
+        if (c < NC) buf[(lo1 + r) * LS + c] = a[q];
       }
     }
     __syncthreads();
-    // y pass: forward FFT along y, store to HBM.  Columns 0 and M2 are real
-    // along y; they travel packed as one complex column c = X[.][0] + i X[.][M2]
-    // in column slot 0, so the pass covers M2 columns.  Their spectra are split
-    // afterwards by Hermitian symmetry.
+    // y pass: forward FFT along y of NC columns, store to HBM; packed, the
+    // spectra of columns 0 and M2 are split afterwards by Hermitian symmetry.
     {
       float2 v[RY][A1];
 #pragma unroll
       for (int r = 0; r < RY; ++r) {
         const int col = r * LY + cy;
-        if (col < M2) {
+        if (col < NC) {
 #pragma unroll
           for (int m = 0; m < A1; ++m) {
             const int i1 = gy + B1 * m;
@@ -699,7 +708,7 @@ __global__ __launch_bounds__(NT, PD_LIG_WAVES) void k_lig_plane_fwd_fast(ScanGeo
 #pragma unroll
       for (int r = 0; r < RY; ++r) {
         const int col = r * LY + cy;
-        if (col < M2) {
+        if (col < NC) {
           four_step_phase1<A1, B1, -1>(v[r], gy, twN1);
 #pragma unroll
           for (int k1 = 0; k1 < A1; ++k1) buf[(k1 * B1 + gy) * LS + col] = v[r][k1];
@@ -731,12 +740,12 @@ __global__ __launch_bounds__(NT, PD_LIG_WAVES) void k_lig_plane_fwd_fast(ScanGeo
 #pragma unroll
         for (int j = 0; j < J1; ++j) {
           const int k1 = gy + B1 * j;
-          if (col < M2 && k1 < A1) {
+          if (col < NC && k1 < A1) {
             float2 x[B1];
 #pragma unroll
             for (int q = 0; q < B1; ++q) x[q] = buf[(k1 * B1 + q) * LS + col];
             rdft<B1, -1>(x);
-            if (col != 0) {
+            if (!packed || col != 0) {
               if constexpr (lin) {
                 const uint32_t o = off8(k1 * H2 + col);
 #pragma unroll
@@ -754,13 +763,15 @@ __global__ __launch_bounds__(NT, PD_LIG_WAVES) void k_lig_plane_fwd_fast(ScanGeo
         }
       }
       __syncthreads();
-      // Y0[k] = (C[k] + conj C[-k]) / 2, YM[k] = (C[k] - conj C[-k]) / (2i)
-      for (int k = tid; k < N1; k += NT) {
-        const float2 c = buf[k * LS + M2], cm = buf[((N1 - k) % N1) * LS + M2];
-        pd_st2(rd, off8(k * H2), 0u, make_float2(0.5f * (c.x + cm.x), 0.5f * (c.y - cm.y)));
-        pd_st2(rd, off8(k * H2 + M2), 0u, make_float2(0.5f * (c.y + cm.y), -0.5f * (c.x - cm.x)));
+      if constexpr (packed) {
+        // Y0[k] = (C[k] + conj C[-k]) / 2, YM[k] = (C[k] - conj C[-k]) / (2i)
+        for (int k = tid; k < N1; k += NT) {
+          const float2 c = buf[k * LS + M2], cm = buf[((N1 - k) % N1) * LS + M2];
+          pd_st2(rd, off8(k * H2), 0u, make_float2(0.5f * (c.x + cm.x), 0.5f * (c.y - cm.y)));
+          pd_st2(rd, off8(k * H2 + M2), 0u, make_float2(0.5f * (c.y + cm.y), -0.5f * (c.x - cm.x)));
+        }
+        __syncthreads();
       }
-      __syncthreads();
     }
   }
 }
