@@ -128,7 +128,7 @@ constexpr int kInvSplitLds = 128 * (kInvSplitN / 2 + 1) * 8;  // 128 rows of the
 static_assert(kInvSplitLds >= 16 * 16 * 64 * 8, "y exchange fits the row buffer");
 const InvSplitFn kInvSplit = k_inv_plane_topk_split<kInvSplitN, kInvSplitN>;
 // ... and the one-kernel ligand plane pass (box rows up to kPlaneSplitRows).
-using LigSplitFn = void (*)(ScanGeom, const float2*, const float2*, const float2*, const int*, int, const float*,
+using LigSplitFn = void (*)(ScanGeom, const float2*, const float2*, const int*, int, const float*,
                             float2*);
 const LigSplitFn kLigSplit = k_lig_plane_split<kInvSplitN, kInvSplitN>;
 static_assert(kInvSplitLds == kPlaneSplitRows * (kInvSplitN / 2 + 1) * 8, "box rows fill the row buffer");
@@ -601,7 +601,7 @@ pd_status launch_fft_path(pd_scan* h, const pd_scan::Lane& ln, int nb, const dou
   {
     Timer tm(h, 1, stream);
     if (h->split && h->lig_fused && g.p[1] <= kPlaneSplitRows) {
-      L(kLigSplit, dim3((g.p[0] + 15) / 16 * 16, nb, g.T), dim3(1024), kInvSplitLds, g, h->tw_y, h->tw_z, h->tw2,
+      L(kLigSplit, dim3((g.p[0] + 15) / 16 * 16, nb, g.T), dim3(1024), kInvSplitLds, g, h->tw_y, h->tw2,
                          ln.cells, h->n_lig, h->w32, ln.L1);
     } else if (h->split || h->lig_rows) {
       L(k_lig_rows, dim3(g.p[0], nb, (g.p[1] + g.zrows - 1) / g.zrows),

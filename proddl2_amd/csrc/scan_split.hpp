@@ -748,7 +748,6 @@ constexpr int kPlaneSplitList = 1536;  // atoms of one plane the one-kernel liga
 
 template <int N1, int N2>
 __global__ __launch_bounds__(1024) void k_lig_plane_split(ScanGeom g, const float2* __restrict__ twN1,
-                                                          const float2* __restrict__ twM2,
                                                           const float2* __restrict__ tw2,
                                                           const int* __restrict__ cells, int n_atoms,
                                                           const float* __restrict__ w, float2* __restrict__ L1) {
@@ -757,7 +756,6 @@ __global__ __launch_bounds__(1024) void k_lig_plane_split(ScanGeom g, const floa
   static_assert(A1 * B1 == N1, "y split");
   constexpr int M2 = N2 / 2, H2 = M2 + 1, LS = H2 | 1, NS = M2 / 64;
   static_assert(M2 % 64 == 0, "whole column sets");
-  (void)twM2;
   extern __shared__ float2 buf[];
   __shared__ float2 s_nyq[N1];
   __shared__ float2 s_tw[N2];
