@@ -119,3 +119,15 @@ def test_graph_and_handle_argument_errors_without_device():
     assert lib.pd_scan_destroy(None) == 0
     assert lib.pd_scan_set_batch(None, 16) == 1
     assert lib.pd_graph_launch(None) == 1
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus above 1 exits with a message before any
+    device work."""
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode != 0
+    assert "one GPU" in (r.stdout + r.stderr)
