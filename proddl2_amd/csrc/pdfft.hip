@@ -955,8 +955,8 @@ pd_status pd_scan_create(const pd_scan_desc* desc, pd_scan** out) {
   g.h = d.grid_step;
   // measured: 256^3 column pass 702 -> 606 ms per 10,000 rotations; 128^3
   // with 16 rotations per block 201.5 -> 189.2 ms per 30,000 (89.2k -> 92.4k
-  // rotations/s)
-  g.col_sync = g.n0 >= 128;
+  // rotations/s); 120^3 104.4k -> 105.6k rotations/s, 96^3 163.5k -> 164.2k
+  g.col_sync = g.n0 >= 96;
   for (int a = 0; a < 3; ++a) {
     centred_axis(d.grid_step, d.n[a], &g.zero_h[a], &g.zsc[a]);
     g.lo[a] = 0;
