@@ -326,8 +326,13 @@ __global__ __launch_bounds__(256) void k_rhat_lanes(ScanGeom g, const float2* __
 // tile's receptor spectra (T * N0 * C complex) are read once into that XCD's
 // L2 and reused nyb times.
 // ---------------------------------------------------------------------------
+// Waves per SIMD the column pass is compiled for: three at 144 and 160 while
+// MZ <= 3 (at most 20 bytes per lane of spills: column pass 144^3 10.4 -> 10.2
+// us per rotation, 160^3 12.6 -> 12.4), else what the registers give (two).
+constexpr int col_waves(int n, int mz) { return (n == 144 || n == 160) && mz <= 3 ? 3 : PD_COL_WAVES; }
+
 template <int N0, int MZ>
-__global__ __launch_bounds__(256, PD_COL_WAVES) void k_col_fast(ScanGeom g, const float2* __restrict__ twN0,
+__global__ __launch_bounds__(256, col_waves(N0, MZ)) void k_col_fast(ScanGeom g, const float2* __restrict__ twN0,
                                                   const float2* __restrict__ L1,
                                                   const float4* __restrict__ RL,
                                                   float2* __restrict__ S1, int bpb, int nrot, int nyb) {
